@@ -1,0 +1,282 @@
+/*
+ * ammsb.h -- C-ABI of the MI355X-native SGRLD a-MMSB sampler (libammsb.so).
+ *
+ * Drop-in boundary for the reference's sequential stochastic sampler
+ *   mcmc::learning::MCMCSamplerStochastic
+ *     (c++/src/mcmc/learning/mcmc_sampler_stochastic.h:72-115,
+ *      c++/src/mcmc/learning/mcmc_sampler_stochastic.cc:9-327)
+ * and its Learner base (c++/src/mcmc/learning/learner.h:24-172).
+ *
+ * State layout is the reference's distributed record layout
+ *   pi[0..K) ++ phi_sum   per node  (mcmc_sampler_stochastic_distr.cc:743-751)
+ * i.e. host arrays exchanged through this ABI are row-major pi[N][K+1] with
+ * slot K holding phi_sum = sum_k phi_ik; theta is [K][2] double; beta is [K]
+ * double.  pi is double for AMMSB_FP64 and float for AMMSB_FP32.
+ *
+ * Conventions: every function returns an int status (AMMSB_OK == 0, negative
+ * on failure); nothing throws across the ABI; ammsb_last_error() returns a
+ * thread-local message for the last failure.  One context owns one HIP stream
+ * and all device buffers; calls on one context are not reentrant, separate
+ * contexts are independent.  Host arrays are owned by the caller and copied.
+ */
+#ifndef AMMSB_H_
+#define AMMSB_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (replace MCMCException, c++/src/mcmc/exception.h:15-84) */
+#define AMMSB_OK 0
+#define AMMSB_ERR_ARG (-1)       /* bad argument / unsupported size          */
+#define AMMSB_ERR_OOM (-2)       /* host or device allocation failed         */
+#define AMMSB_ERR_HIP (-3)       /* HIP runtime error                        */
+#define AMMSB_ERR_INVARIANT (-4) /* sampler invariant violated (e.g. minibatch
+                                    exceeds buffer, network.cc:399-404)      */
+#define AMMSB_ERR_STATE (-5)     /* call order (e.g. step before init)       */
+
+/* ---- sampling strategy (c++/src/mcmc/types.h:11-16) */
+enum ammsb_strategy {
+  AMMSB_STRATIFIED_RANDOM_NODE = 0,
+  AMMSB_STRATIFIED_RANDOM_NODE_LINKS = 1,
+  AMMSB_STRATIFIED_RANDOM_NODE_NONLINKS = 2,
+  AMMSB_RANDOM_EDGE = 3
+};
+
+/* ---- Float precision (c++/src/mcmc/config.h.in:14-20 picks float or double
+ * for everything; AMMSB_FP32 here is the "precision mode": FP32 bulk
+ * arithmetic with FP64 perplexity log-sum, theta/beta gradient accumulation
+ * and pi renormalisation). */
+enum ammsb_precision { AMMSB_FP64 = 0, AMMSB_FP32 = 1 };
+
+/* ---- options: every --mcmc.* flag of c++/src/mcmc/options.h:139-212 that
+ * the sequential sampler reads, with the same defaults.  Zero means "derive"
+ * exactly where the reference derives:
+ *   a      = b^-c            (options.h:257-259)
+ *   alpha  = 1/K             (options.h:260-262)
+ *   interval = 50, stats_print_interval = 65536
+ *                            (mcmc_sampler_stochastic.cc:17-26)
+ *   num_node_sample = N/50, mini_batch_size = N/10
+ *                            (mcmc_sampler_stochastic.cc:35-44)
+ *   held_out_ratio = 0.1     (learner.cc:87-91)                             */
+typedef struct ammsb_options {
+  double alpha;                 /* --mcmc.alpha (0)                */
+  double eta0;                  /* --mcmc.eta0 (1)                 */
+  double eta1;                  /* --mcmc.eta1 (1)                 */
+  double epsilon;               /* --mcmc.epsilon (1e-7)           */
+  double a;                     /* --mcmc.a (0 -> b^-c)            */
+  double b;                     /* --mcmc.b (1024)                 */
+  double c;                     /* --mcmc.c (0.5)                  */
+  int64_t K;                    /* --mcmc.K (300)                  */
+  int64_t mini_batch_size;      /* --mcmc.mini-batch-size (0)      */
+  int64_t num_node_sample;      /* --mcmc.num-node-sample (0)      */
+  int32_t strategy;             /* --mcmc.strategy                 */
+  int64_t sampler_max_source;   /* --mcmc.sampler.max-source (1)   */
+  int64_t max_iteration;        /* --mcmc.max-iteration (10000000) */
+  int64_t interval;             /* --mcmc.interval (0 -> 50)       */
+  int64_t stats_print_interval; /* --mcmc.stats (0 -> 65536)       */
+  int64_t num_updates;          /* --mcmc.num-updates (1000)       */
+  double held_out_ratio;        /* --mcmc.held-out-ratio (0 -> 0.1) */
+  int32_t random_seed;          /* --mcmc.seed (42)                */
+  double convergence_threshold; /* --mcmc.convergence (1e-12)      */
+  int32_t precision;            /* ammsb_precision (AMMSB_FP64)    */
+  int32_t device;               /* HIP device ordinal (0)          */
+  int32_t verbose;              /* echo config to stderr like
+                                   learner.cc:105-120 (0)          */
+} ammsb_options;
+
+/* Undirected graph as a symmetric CSR: row i lists the neighbours of i in
+ * ascending order, no self loops, no duplicates; cols has 2*num_edges
+ * entries.  Replaces NetworkGraph (c++/src/mcmc/data.h:135-313). */
+typedef struct ammsb_graph {
+  int32_t N;
+  int64_t num_edges;
+  const int64_t *row_ptr; /* N+1 */
+  const int32_t *cols;    /* 2*num_edges */
+} ammsb_graph;
+
+/* Held-out and test sets (EdgeMap, data.h:315-317): edges (a,b) with a<b,
+ * label 1 = link, 0 = non-link.  Replaces Network::held_out_map/test_map
+ * (network.h:326-327). */
+typedef struct ammsb_split {
+  int64_t held_out_count;
+  const int32_t *held_out_edges; /* [held_out_count][2] */
+  const uint8_t *held_out_labels;
+  int64_t test_count;
+  const int32_t *test_edges; /* [test_count][2] */
+  const uint8_t *test_labels;
+} ammsb_split;
+
+/* Derived quantities of a context (read-only view). */
+typedef struct ammsb_info {
+  int32_t N;
+  int64_t K;
+  int64_t num_edges;
+  int64_t ld;              /* device row stride of pi ++ phi_sum (elements) */
+  int64_t mini_batch_size; /* after derivation */
+  int64_t num_node_sample; /* n; n+1 neighbours are drawn per node */
+  int64_t interval;
+  int64_t stats_print_interval;
+  int64_t max_minibatch_edges;
+  int64_t max_minibatch_nodes;
+  int64_t held_out_count;
+  int64_t test_count;
+  int64_t step_count;    /* Learner::step_count (starts at 1) */
+  int64_t average_count; /* Learner::average_count (starts at 1) */
+  double alpha, a, b, c, epsilon, eta0, eta1;
+  int32_t precision;
+  int32_t strategy;
+  int64_t sampler_max_source;
+  int64_t max_iteration;
+} ammsb_info;
+
+typedef struct ammsb_ctx ammsb_ctx;
+
+/* ---- library ----------------------------------------------------------- */
+const char *ammsb_last_error(void);
+const char *ammsb_version(void);
+/* Fill with the reference defaults (options.h:139-212). */
+int ammsb_options_default(ammsb_options *opts);
+
+/* ---- host-side graph tools (no GPU needed) ----------------------------- */
+/* Synthetic generators (see ammsb_gen_params).  The library allocates
+ * *row_ptr (N+1) and *cols (2E); release both with ammsb_free(). */
+enum ammsb_gen_kind {
+  /* pi_a ~ Dir(gen_alpha), beta_k ~ U(beta_lo, beta_hi); pair (a,b) links
+   * with prob sum_k pi_ak pi_bk beta_k + delta (1 - sum_k pi_ak pi_bk).
+   * O(N^2 K): small N only. */
+  AMMSB_GEN_PLANTED = 0,
+  /* each node joins 1..max_comms communities (weights Dir(gen_alpha));
+   * Erdos-Renyi inside each community with p set for mean_degree, plus
+   * round(delta * N(N-1)/2) uniform background pairs.  O(N + E). */
+  AMMSB_GEN_COMMUNITY = 1
+};
+typedef struct ammsb_gen_params {
+  int32_t kind;
+  int32_t N;
+  int32_t K;
+  double gen_alpha;   /* Dirichlet concentration */
+  double beta_lo;     /* PLANTED */
+  double beta_hi;     /* PLANTED */
+  double delta;       /* background link probability (1e-7) */
+  double mean_degree; /* COMMUNITY */
+  int32_t max_comms;  /* COMMUNITY (3) */
+  uint64_t seed;
+} ammsb_gen_params;
+int ammsb_graph_generate(const ammsb_gen_params *p, int64_t **row_ptr,
+                         int32_t **cols, int64_t *num_edges);
+
+/* Held-out/test split with the reference's semantics
+ * (network.cc:503-614): held_out_size = ratio*E, p = held_out_size/2;
+ * held-out = p random edges + p random non-edges; test = p random edges not
+ * in held-out + p non-edges not in either.  Arrays allocated by the
+ * library; release with ammsb_free(). */
+int ammsb_split_generate(const ammsb_graph *g, double held_out_ratio,
+                         uint64_t seed, int32_t **held_out_edges,
+                         uint8_t **held_out_labels, int64_t *held_out_count,
+                         int32_t **test_edges, uint8_t **test_labels,
+                         int64_t *test_count);
+void ammsb_free(void *p);
+/* One Philox4x32-10 block of the library's counter-based stream:
+ * out = philox(ctr[4], key[2]) (Salmon et al., SC'11).  The stream replaces
+ * the reference's xorshift128+ generator (c++/src/mcmc/random.cc:136-176);
+ * exported so callers can replay any draw of the samplers. */
+void ammsb_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ---- context (Learner / MCMCSamplerStochastic) ------------------------- */
+/* Replaces MCMCSamplerStochastic(const Options&) + LoadNetwork()
+ * (mcmc_sampler_stochastic.cc:9-44, learner.cc:61-97): copies the graph,
+ * builds the device CSR and the held-out∪test hash, derives defaults and
+ * allocates the HBM state.  split may be NULL: the library then draws the
+ * split itself with ammsb_split_generate(ratio, seed). */
+int ammsb_create(const ammsb_options *opts, const ammsb_graph *g,
+                 const ammsb_split *split, ammsb_ctx **out);
+int ammsb_destroy(ammsb_ctx *ctx);
+int ammsb_get_info(const ammsb_ctx *ctx, ammsb_info *info);
+
+/* Replaces MCMCSamplerStochastic::init() state draw
+ * (mcmc_sampler_stochastic.cc:54-76): theta ~ Gamma(eta0, eta1) per entry,
+ * beta = theta[:,1]/rowsum, phi ~ Gamma(1,1), pi = phi/rowsum.  Counter-based
+ * draws keyed by seed. */
+int ammsb_init_state(ammsb_ctx *ctx, uint64_t seed);
+/* Upload / download the state. pi: [N][K+1] (double or float per precision),
+ * theta: [K][2] double.  beta is recomputed from theta on upload. */
+int ammsb_set_state(ammsb_ctx *ctx, const void *pi, const double *theta);
+int ammsb_get_state(ammsb_ctx *ctx, void *pi, double *theta, double *beta);
+/* Rows of pi ++ phi_sum for a list of nodes ([n][K+1]). */
+int ammsb_get_rows(ammsb_ctx *ctx, const int32_t *nodes, int64_t n,
+                   void *out);
+/* Learner::step_count / average_count setters (tests, resumption). */
+int ammsb_set_counters(ammsb_ctx *ctx, int64_t step_count,
+                       int64_t average_count);
+
+/* ---- the loop ---------------------------------------------------------- */
+/* iters full SGRLD iterations on the device, each exactly the body of
+ * MCMCSamplerStochastic::run() (mcmc_sampler_stochastic.cc:131-174) without
+ * the perplexity check: minibatch, nodes_in_batch, eps_t, neighbour
+ * sampling + update_phi per node, update_pi, update_beta, step_count++.
+ * Asynchronous w.r.t. the host except for error checks when sync != 0. */
+int ammsb_step(ammsb_ctx *ctx, int64_t iters, int32_t sync);
+/* Learner::cal_perplexity_held_out (learner.cc:126-183): updates the
+ * running average per held-out edge, returns -mean log, increments
+ * average_count. */
+int ammsb_perplexity(ammsb_ctx *ctx, double *ppx);
+/* MCMCSamplerStochastic::run() (mcmc_sampler_stochastic.cc:93-183): the
+ * loop with perplexity every `interval` steps, the convergence test, the
+ * reference's stdout line
+ *   "average_count is: <c> step count: <t> time: <s> perplexity for hold
+ *    out set: <x>"
+ * and the per-phase timer table.  max_steps <= 0 means max_iteration.
+ * If ppx_out/ppx_steps are non-NULL, up to cap checkpoints are stored and
+ * *n_ppx receives how many were produced. */
+int ammsb_run(ammsb_ctx *ctx, int64_t max_steps, int32_t print,
+              double *ppx_out, int64_t *ppx_steps, int64_t cap,
+              int64_t *n_ppx);
+/* Per-phase accumulated device time in seconds and tick counts, in the
+ * order of mcmc_sampler_stochastic.cc:95-102: outer, perplexity,
+ * sample_mini_batch, nodes_in_mini_batch, sample_neighbor_nodes,
+ * update_phi, update_pi, update_beta.  secs[8], ticks[8]. */
+int ammsb_timers(ammsb_ctx *ctx, double *secs, int64_t *ticks);
+int ammsb_enable_timers(ammsb_ctx *ctx, int32_t on);
+
+/* ---- single operators with injected draws (parity) --------------------- */
+/* update_phi for n_nodes distinct nodes followed by update_pi for the same
+ * nodes: MCMCSamplerStochastic::update_phi (mcmc_sampler_stochastic.cc:
+ * 269-313) in the pi ++ phi_sum form of update_phi_node
+ * (mcmc_sampler_stochastic_distr.cc:1158-1227) + pi_from_phi (:743-751).
+ * nbrs: [n_nodes][n_nb] neighbour ids (a neighbour equal to the node is
+ * skipped as in :275-277); noise: [n_nodes][K] (same precision as pi) or
+ * NULL for the Philox noise of iteration step_count. */
+int ammsb_update_phi(ammsb_ctx *ctx, const int32_t *nodes, int64_t n_nodes,
+                     const int32_t *nbrs, int64_t n_nb, const void *noise,
+                     double eps_t);
+/* MCMCSamplerStochastic::update_beta (mcmc_sampler_stochastic.cc:200-267):
+ * edges [E][2], y [E] in {0,1}, noise [K][2] double (NULL = Philox). */
+int ammsb_update_beta(ammsb_ctx *ctx, const int32_t *edges, const uint8_t *y,
+                      int64_t E, double scale, const double *noise,
+                      double eps_t);
+
+/* ---- device sampler outputs (bit-exact parity against a host replica) -- */
+/* Runs the device minibatch sampler, nodes_in_batch and neighbour sampler
+ * for step_count == step (state is not modified) and copies out:
+ * flag (0 non-links, 1 links), scale, the minibatch edges [n_edges][2] in
+ * sampler order, the unique nodes (unordered), and per node its n+1
+ * neighbours [n_nodes][n+1] with bit 31 set when (node, nbr) is a link.
+ * Buffers must hold max_minibatch_edges / max_minibatch_nodes entries. */
+int ammsb_debug_sample(ammsb_ctx *ctx, int64_t step, int32_t *flag,
+                       double *scale, int32_t *edges, int64_t *n_edges,
+                       int32_t *nodes, int64_t *n_nodes, int32_t *nbrs);
+
+/* Device pointers of the resident state for interop (torch DLPack-free
+ * views): pi ([N][ld] of the precision type), theta ([K][2] double),
+ * beta ([K] double). */
+int ammsb_device_ptrs(ammsb_ctx *ctx, void **pi, void **theta, void **beta);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AMMSB_H_ */

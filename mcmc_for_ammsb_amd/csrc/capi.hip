@@ -1,0 +1,1118 @@
+// Host driver and C-ABI of libammsb.so.
+//
+// The context is the MI355X-native counterpart of MCMCSamplerStochastic +
+// Learner (c++/src/mcmc/learning/mcmc_sampler_stochastic.{h,cc},
+// learner.{h,cc}): options and derived defaults, HBM-resident state
+// (pi ++ phi_sum rows, theta, beta), graph (CSR + held-out∪test hash),
+// and the run() loop with its perplexity schedule, convergence test,
+// stdout log line and per-phase timers.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "../../include/ammsb.h"
+#include "host_util.h"
+#include "kernels.h"
+
+namespace ammsb {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* get_error() { return g_err.c_str(); }
+
+namespace {
+
+enum Phase {
+  PH_OUTER = 0,
+  PH_PERPLEXITY,
+  PH_MINIBATCH,
+  PH_NODES,
+  PH_NEIGHBORS,
+  PH_UPDATE_PHI,
+  PH_UPDATE_PI,
+  PH_UPDATE_BETA,
+  PH_COUNT
+};
+const char* kPhaseNames[PH_COUNT] = {"  outer",           "  perplexity",
+                                     "  sample_mini_batch", "  nodes_in_mini_batch",
+                                     "  sample_neighbor_nodes", "  update_phi",
+                                     "  update_pi",         "  update_beta"};
+
+struct PendingEvent {
+  int phase;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+}  // namespace ammsb
+
+using namespace ammsb;
+
+struct ammsb_ctx {
+  ammsb_options opt;
+  int32_t N = 0;
+  int64_t E = 0;
+  int32_t K = 0;
+  int64_t ld = 0;
+  int prec = 0;
+  size_t fsize = 8;
+  int64_t m = 0, n = 0, nnb = 0;
+  int64_t interval = 50, stats_interval = 65536;
+  int64_t edge_cap = 0, node_cap = 0;
+  int64_t max_degree = 0;
+  hipStream_t stream = nullptr;
+  Key key{};
+  DevHyper hp{};
+  // device graph
+  int64_t* d_row_ptr = nullptr;
+  int32_t* d_cols = nullptr;
+  uint64_t* d_hkeys = nullptr;
+  uint8_t* d_hvals = nullptr;
+  uint64_t hmask = 0;
+  // held-out set (perplexity)
+  int64_t H = 0, T = 0;
+  int32_t* d_ho_edges = nullptr;
+  uint8_t* d_ho_labels = nullptr;
+  double* d_R = nullptr;
+  // state
+  char* d_pi = nullptr;       // [N][ld] of the precision type
+  double* d_theta = nullptr;
+  double* d_beta = nullptr;
+  DevState* d_st = nullptr;
+  // iteration scratch
+  int32_t* d_mb_edges = nullptr;
+  uint8_t* d_mb_y = nullptr;
+  int32_t* d_nodes = nullptr;
+  int32_t* d_nbrs = nullptr;
+  char* d_staging = nullptr;  // [node_cap][ld]
+  int2* d_mbtable = nullptr;
+  int64_t mbtable_cap = 0;
+  uint32_t* d_stamp = nullptr;
+  int2* d_nbr_gtable = nullptr;
+  int32_t nbr_tcap = 0, nbr_use_lds = 1, nbr_gslots = 0;
+  double* d_beta_partials = nullptr;
+  int32_t beta_blocks = 1;
+  double* d_ppx_partials = nullptr;
+  int32_t ppx_blocks = 1;
+  // host mirrors of Learner counters
+  int64_t step_count = 1;
+  int64_t average_count = 1;
+  std::deque<double> ppx_cb;  // boost::circular_buffer<Float>(10), learner.cc:8
+  bool state_ready = false;
+  // timers
+  bool timers_on = false;
+  double t_secs[PH_COUNT] = {0};
+  int64_t t_ticks[PH_COUNT] = {0};
+  std::vector<PendingEvent> pending;
+  std::vector<hipEvent_t> event_pool;
+};
+
+namespace {
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess) {                                                        \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                \
+      return AMMSB_ERR_HIP;                                                        \
+    }                                                                              \
+  } while (0)
+
+template <typename T>
+int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc(") + std::to_string(count * sizeof(T)) +
+              " bytes): " + hipGetErrorString(e));
+    *p = nullptr;
+    return AMMSB_ERR_OOM;
+  }
+  return AMMSB_OK;
+}
+
+#define ALLOC_TRY(p, count)           \
+  do {                                \
+    int _rc = dalloc(&(p), (count));  \
+    if (_rc != AMMSB_OK) return _rc;  \
+  } while (0)
+
+uint64_t pow2_at_least(uint64_t x) {
+  uint64_t c = 1;
+  while (c < x) c <<= 1;
+  return c;
+}
+
+DevGraph dev_graph(const ammsb_ctx* c) {
+  DevGraph g;
+  g.N = c->N;
+  g.E = c->E;
+  g.row_ptr = c->d_row_ptr;
+  g.cols = c->d_cols;
+  g.hkeys = c->d_hkeys;
+  g.hvals = c->d_hvals;
+  g.hmask = c->hmask;
+  return g;
+}
+
+hipEvent_t take_event(ammsb_ctx* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int flush_timers(ammsb_ctx* c) {
+  if (c->pending.empty()) return AMMSB_OK;
+  HIP_TRY(hipEventSynchronize(c->pending.back().b));
+  for (auto& p : c->pending) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
+    c->t_secs[p.phase] += ms * 1e-3;
+    c->t_ticks[p.phase] += 1;
+    c->event_pool.push_back(p.a);
+    c->event_pool.push_back(p.b);
+  }
+  c->pending.clear();
+  return AMMSB_OK;
+}
+
+struct PhaseScope {
+  ammsb_ctx* c;
+  int phase;
+  hipEvent_t a = nullptr;
+  PhaseScope(ammsb_ctx* c_, int ph) : c(c_), phase(ph) {
+    if (c->timers_on) {
+      a = take_event(c);
+      if (a) (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~PhaseScope() {
+    if (c->timers_on && a) {
+      hipEvent_t b = take_event(c);
+      if (b) {
+        (void)hipEventRecord(b, c->stream);
+        c->pending.push_back({phase, a, b});
+        if (c->pending.size() > 4096) flush_timers(c);
+      }
+    }
+  }
+};
+
+// ---- one SGRLD iteration on the device (mcmc_sampler_stochastic.cc:131-174)
+int enqueue_iteration(ammsb_ctx* c) {
+  const DevGraph g = dev_graph(c);
+  {
+    PhaseScope ps(c, PH_MINIBATCH);
+    SamplerArgs sa;
+    sa.g = g;
+    sa.st = c->d_st;
+    sa.key = c->key;
+    sa.strategy = c->opt.strategy;
+    sa.m = c->m;
+    sa.max_source = c->opt.sampler_max_source;
+    sa.edge_cap = c->edge_cap;
+    sa.mb_edges = c->d_mb_edges;
+    sa.mb_y = c->d_mb_y;
+    sa.nodes = c->d_nodes;
+    sa.node_cap = c->node_cap;
+    sa.table = c->d_mbtable;
+    sa.table_cap = c->mbtable_cap;
+    sa.stamp = c->d_stamp;
+    sa.step_override = -1;
+    HIP_TRY(launch_minibatch(sa, c->stream));
+  }
+  {
+    PhaseScope ps(c, PH_NEIGHBORS);
+    NeighborArgs na;
+    na.g = g;
+    na.st = c->d_st;
+    na.key = c->key;
+    na.nodes = c->d_nodes;
+    na.nnb = (int32_t)c->nnb;
+    na.nbrs = c->d_nbrs;
+    na.gtable = c->d_nbr_gtable;
+    na.tcap = c->nbr_tcap;
+    na.use_lds = c->nbr_use_lds;
+    na.step_override = -1;
+    na.gtable_slots = c->nbr_gslots;
+    HIP_TRY(launch_neighbors(na, (int32_t)c->node_cap, c->stream));
+  }
+  {
+    PhaseScope ps(c, PH_UPDATE_PHI);
+    PhiArgs pa;
+    pa.pi = c->d_pi;
+    pa.ld = c->ld;
+    pa.K = c->K;
+    pa.N = c->N;
+    pa.nodes = c->d_nodes;
+    pa.count_dev = &c->d_st->node_count;
+    pa.count_host = 0;
+    pa.nbrs = c->d_nbrs;
+    pa.nnb = (int32_t)c->nnb;
+    pa.beta = c->d_beta;
+    pa.st = c->d_st;
+    pa.hp = c->hp;
+    pa.eps_override = -1.0;
+    pa.noise = nullptr;
+    pa.key = c->key;
+    pa.staging = c->d_staging;
+    HIP_TRY(launch_update_phi(pa, c->prec, (int32_t)c->node_cap, c->stream));
+  }
+  {
+    PhaseScope ps(c, PH_UPDATE_PI);
+    ScatterArgs sc;
+    sc.pi = c->d_pi;
+    sc.ld = c->ld;
+    sc.K = c->K;
+    sc.nodes = c->d_nodes;
+    sc.count_dev = &c->d_st->node_count;
+    sc.count_host = 0;
+    sc.staging = c->d_staging;
+    HIP_TRY(launch_scatter(sc, c->prec, (int32_t)c->node_cap, c->stream));
+  }
+  {
+    PhaseScope ps(c, PH_UPDATE_BETA);
+    BetaArgs ba;
+    ba.pi = c->d_pi;
+    ba.ld = c->ld;
+    ba.K = c->K;
+    ba.edges = c->d_mb_edges;
+    ba.y = c->d_mb_y;
+    ba.count_dev = &c->d_st->mb_count;
+    ba.count_host = 0;
+    ba.beta = c->d_beta;
+    ba.theta = c->d_theta;
+    ba.epsilon = c->hp.epsilon;
+    ba.partials = c->d_beta_partials;
+    ba.blocks = c->beta_blocks;
+    HIP_TRY(launch_beta(ba, c->prec, c->stream));
+    ThetaArgs ta;
+    ta.theta = c->d_theta;
+    ta.beta = c->d_beta;
+    ta.K = c->K;
+    ta.partials = c->d_beta_partials;
+    ta.blocks = c->beta_blocks * kBetaWaves;
+    ta.st = c->d_st;
+    ta.hp = c->hp;
+    ta.eps_override = -1.0;
+    ta.scale_override = -1.0;
+    ta.noise = nullptr;
+    ta.key = c->key;
+    ta.bump_step = 1;
+    HIP_TRY(launch_theta(ta, c->stream));
+  }
+  return AMMSB_OK;
+}
+
+int read_devstate(ammsb_ctx* c, DevState* h) {
+  HIP_TRY(hipMemcpyAsync(h, c->d_st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AMMSB_OK;
+}
+
+int check_devstate(ammsb_ctx* c) {
+  DevState h;
+  int rc = read_devstate(c, &h);
+  if (rc != AMMSB_OK) return rc;
+  if (h.error) {
+    std::string msg = "device sampler error:";
+    if (h.error & ERR_NONLINK_ATTEMPTS) msg += " non-link attempt budget exceeded;";
+    if (h.error & ERR_LINK_ROUNDS) msg += " link sampler found no training edges;";
+    if (h.error & ERR_NEIGHBOR_ATTEMPTS) msg += " neighbour attempt budget exceeded;";
+    if (h.error & ERR_MB_OVERFLOW) msg += " #edge in minibatch exceeds buffer size upper bound;";
+    if (h.error & ERR_BAD_STRATEGY) msg += " invalid sampling strategy;";
+    set_error(msg);
+    return AMMSB_ERR_INVARIANT;
+  }
+  if (h.step != c->step_count) {
+    set_error("device step_count diverged from host mirror");
+    return AMMSB_ERR_STATE;
+  }
+  return AMMSB_OK;
+}
+
+int do_perplexity(ammsb_ctx* c, double* out) {
+  PpxArgs p;
+  p.pi = c->d_pi;
+  p.ld = c->ld;
+  p.K = c->K;
+  p.edges = c->d_ho_edges;
+  p.labels = c->d_ho_labels;
+  p.H = c->H;
+  p.R = c->d_R;
+  p.avg_count = (double)c->average_count;
+  p.beta = c->d_beta;
+  p.epsilon = c->hp.epsilon;
+  p.partials = c->d_ppx_partials;
+  p.blocks = c->ppx_blocks;
+  p.st = c->d_st;
+  {
+    PhaseScope ps(c, PH_PERPLEXITY);
+    HIP_TRY(launch_perplexity(p, c->prec, c->stream));
+  }
+  DevState h;
+  int rc = read_devstate(c, &h);
+  if (rc != AMMSB_OK) return rc;
+  c->average_count += 1;  // learner.cc:180
+  *out = h.ppx;
+  return AMMSB_OK;
+}
+
+void print_stats(ammsb_ctx* c, FILE* f) {
+  flush_timers(c);
+  // timer.cc:18-56 tabular layout
+  fprintf(f, "%-37s%13s%9s%15s\n", "timer", "total (s)", "ticks", "per tick (us)");
+  for (int p = 0; p < PH_COUNT; ++p) {
+    if (c->t_ticks[p] == 0) {
+      fprintf(f, "%-36s<unused>\n", kPhaseNames[p]);
+    } else {
+      fprintf(f, "%-36s %12.3f %8lld %14.3f\n", kPhaseNames[p], c->t_secs[p],
+              (long long)c->t_ticks[p], c->t_secs[p] * 1e6 / (double)c->t_ticks[p]);
+    }
+  }
+  fflush(f);
+}
+
+bool is_converged(const ammsb_ctx* c) {
+  // learner.cc:130-136
+  const size_t n = c->ppx_cb.size();
+  if (n < 2) return false;
+  return std::abs(c->ppx_cb[n - 1] - c->ppx_cb[n - 2]) / c->ppx_cb[n - 2] <=
+         c->opt.convergence_threshold;
+}
+
+void free_ctx(ammsb_ctx* c) {
+  if (!c) return;
+  void* ptrs[] = {c->d_row_ptr, c->d_cols,    c->d_hkeys,    c->d_hvals,     c->d_ho_edges,
+                  c->d_ho_labels, c->d_R,     c->d_pi,       c->d_theta,     c->d_beta,
+                  c->d_st,      c->d_mb_edges, c->d_mb_y,    c->d_nodes,     c->d_nbrs,
+                  c->d_staging, c->d_mbtable, c->d_stamp,    c->d_nbr_gtable, c->d_beta_partials,
+                  c->d_ppx_partials};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto& p : c->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_split* split,
+                ammsb_ctx* c) {
+  c->opt = *opts;
+  ammsb_options& o = c->opt;
+  if (!g || !g->row_ptr || !g->cols || g->N < 2) {
+    set_error("ammsb_create: bad graph");
+    return AMMSB_ERR_ARG;
+  }
+  if (o.K < 1) {
+    set_error("ammsb_create: K must be >= 1");
+    return AMMSB_ERR_ARG;
+  }
+  if (o.precision != AMMSB_FP64 && o.precision != AMMSB_FP32) {
+    set_error("ammsb_create: unknown precision");
+    return AMMSB_ERR_ARG;
+  }
+  if (o.strategy < 0 || o.strategy > 3) {
+    set_error("ammsb_create: Invalid sampling strategy");
+    return AMMSB_ERR_ARG;
+  }
+  if (o.strategy == AMMSB_RANDOM_EDGE) {
+    set_error("ammsb_create: strategy random-edge is not implemented on the device path");
+    return AMMSB_ERR_ARG;
+  }
+  c->N = g->N;
+  c->E = g->num_edges;
+  c->K = (int32_t)o.K;
+  c->prec = o.precision;
+  c->fsize = c->prec == AMMSB_FP64 ? 8 : 4;
+  int lpr, ch;
+  if (!row_config(c->K, c->prec, &lpr, &ch)) {
+    set_error("ammsb_create: K too large for the device kernels (max 2048 FP64 / 4096 FP32)");
+    return AMMSB_ERR_ARG;
+  }
+  const int rpw = 64 / lpr;
+  // row stride: K+1 elements rounded up to 128 bytes
+  const int64_t per128 = 128 / (int64_t)c->fsize;
+  c->ld = ((int64_t)c->K + 1 + per128 - 1) / per128 * per128;
+  // derived defaults (options.h:257-262, mcmc_sampler_stochastic.cc:17-44,
+  // learner.cc:87-91)
+  if (o.a == 0.0) o.a = std::pow(o.b, -o.c);
+  if (o.alpha == 0.0) o.alpha = 1.0 / (double)o.K;
+  c->interval = o.interval == 0 ? 50 : o.interval;
+  c->stats_interval = o.stats_print_interval == 0 ? 64 * 1024 : o.stats_print_interval;
+  c->n = o.num_node_sample == 0 ? c->N / 50 : o.num_node_sample;
+  c->m = o.mini_batch_size == 0 ? c->N / 10 : o.mini_batch_size;
+  if (o.held_out_ratio == 0.0) o.held_out_ratio = 0.1;
+  if (c->n < 1 || c->n + 1 > c->N - 1) {
+    set_error("ammsb_create: num_node_sample must be in [1, N-2]");
+    return AMMSB_ERR_ARG;
+  }
+  if (c->m < 1 || c->m > (int64_t)c->N - 1) {
+    set_error("ammsb_create: mini_batch_size must be in [1, N-1]");
+    return AMMSB_ERR_ARG;
+  }
+  if (o.sampler_max_source < 1 || o.sampler_max_source > 1024) {
+    set_error("ammsb_create: sampler.max-source must be in [1, 1024]");
+    return AMMSB_ERR_ARG;
+  }
+  c->nnb = c->n + 1;
+  c->key = make_key((uint64_t)(uint32_t)o.random_seed);
+  c->hp.alpha = o.alpha;
+  c->hp.epsilon = o.epsilon;
+  c->hp.a = o.a;
+  c->hp.b = o.b;
+  c->hp.c = o.c;
+  c->hp.eta0 = o.eta0;
+  c->hp.eta1 = o.eta1;
+  c->hp.Nn = (1.0 * (double)c->N) / (double)c->n;
+  if (o.precision == AMMSB_FP32) c->hp.Nn = (double)(float)((1.0f * (float)c->N) / (float)c->n);
+
+  HIP_TRY(hipSetDevice(o.device));
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+
+  // ---- graph
+  for (int32_t i = 0; i < c->N; ++i)
+    c->max_degree = std::max<int64_t>(c->max_degree, g->row_ptr[i + 1] - g->row_ptr[i]);
+  if (g->row_ptr[c->N] != 2 * c->E) {
+    set_error("ammsb_create: row_ptr[N] != 2*num_edges");
+    return AMMSB_ERR_ARG;
+  }
+  ALLOC_TRY(c->d_row_ptr, (size_t)c->N + 1);
+  ALLOC_TRY(c->d_cols, (size_t)(2 * c->E));
+  HIP_TRY(hipMemcpy(c->d_row_ptr, g->row_ptr, sizeof(int64_t) * (c->N + 1), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_cols, g->cols, sizeof(int32_t) * 2 * c->E, hipMemcpyHostToDevice));
+
+  // ---- split
+  int32_t *ho = nullptr, *te = nullptr;
+  uint8_t *hol = nullptr, *tel = nullptr;
+  int64_t hc = 0, tc = 0;
+  bool own_split = false;
+  if (split) {
+    ho = const_cast<int32_t*>(split->held_out_edges);
+    hol = const_cast<uint8_t*>(split->held_out_labels);
+    hc = split->held_out_count;
+    te = const_cast<int32_t*>(split->test_edges);
+    tel = const_cast<uint8_t*>(split->test_labels);
+    tc = split->test_count;
+  } else {
+    int rc = ammsb_split_generate(g, o.held_out_ratio, (uint64_t)(uint32_t)o.random_seed, &ho,
+                                  &hol, &hc, &te, &tel, &tc);
+    if (rc != AMMSB_OK) return rc;
+    own_split = true;
+  }
+  c->H = hc;
+  c->T = tc;
+  EdgeHashHost hh;
+  hh.init(hc + tc);
+  for (int64_t i = 0; i < hc; ++i) {
+    const int32_t a = std::min(ho[2 * i], ho[2 * i + 1]), b = std::max(ho[2 * i], ho[2 * i + 1]);
+    hh.insert(edge_key((uint32_t)a, (uint32_t)b), (uint8_t)(1 | (hol[i] ? 4 : 0)));
+  }
+  for (int64_t i = 0; i < tc; ++i) {
+    const int32_t a = std::min(te[2 * i], te[2 * i + 1]), b = std::max(te[2 * i], te[2 * i + 1]);
+    hh.insert(edge_key((uint32_t)a, (uint32_t)b), (uint8_t)(2 | (tel[i] ? 4 : 0)));
+  }
+  c->hmask = hh.mask;
+  ALLOC_TRY(c->d_hkeys, hh.keys.size());
+  ALLOC_TRY(c->d_hvals, hh.vals.size());
+  HIP_TRY(hipMemcpy(c->d_hkeys, hh.keys.data(), 8 * hh.keys.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_hvals, hh.vals.data(), hh.vals.size(), hipMemcpyHostToDevice));
+  ALLOC_TRY(c->d_ho_edges, (size_t)(2 * hc));
+  ALLOC_TRY(c->d_ho_labels, (size_t)hc);
+  ALLOC_TRY(c->d_R, (size_t)hc);
+  if (hc) {
+    std::vector<int32_t> hoe(2 * hc);
+    for (int64_t i = 0; i < hc; ++i) {
+      hoe[2 * i] = std::min(ho[2 * i], ho[2 * i + 1]);
+      hoe[2 * i + 1] = std::max(ho[2 * i], ho[2 * i + 1]);
+    }
+    HIP_TRY(hipMemcpy(c->d_ho_edges, hoe.data(), 8 * hc, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_ho_labels, hol, hc, hipMemcpyHostToDevice));
+  }
+  if (own_split) {
+    free(ho);
+    free(hol);
+    free(te);
+    free(tel);
+  }
+  HIP_TRY(launch_fill_f64(c->d_R, 0.0, std::max<int64_t>(hc, 1), c->stream));
+
+  // ---- state
+  ALLOC_TRY(c->d_pi, (size_t)c->N * c->ld * c->fsize);
+  ALLOC_TRY(c->d_theta, (size_t)2 * c->K);
+  ALLOC_TRY(c->d_beta, (size_t)c->K);
+  ALLOC_TRY(c->d_st, 1);
+  DevState h0{};
+  h0.step = 1;
+  HIP_TRY(hipMemcpy(c->d_st, &h0, sizeof(DevState), hipMemcpyHostToDevice));
+
+  // ---- iteration scratch (sizes: network.cc:172-201)
+  c->edge_cap = c->m + c->max_degree + 1;
+  c->node_cap = std::min<int64_t>(c->N, c->edge_cap + std::min<int64_t>(o.sampler_max_source, c->N) + 1);
+  ALLOC_TRY(c->d_mb_edges, (size_t)(2 * c->edge_cap));
+  ALLOC_TRY(c->d_mb_y, (size_t)c->edge_cap);
+  ALLOC_TRY(c->d_nodes, (size_t)c->node_cap);
+  ALLOC_TRY(c->d_nbrs, (size_t)(c->node_cap * c->nnb));
+  ALLOC_TRY(c->d_staging, (size_t)(c->node_cap * c->ld * c->fsize));
+  c->mbtable_cap = (int64_t)pow2_at_least((uint64_t)(2 * (c->m + 1024)));
+  ALLOC_TRY(c->d_mbtable, (size_t)c->mbtable_cap);
+  ALLOC_TRY(c->d_stamp, (size_t)c->N);
+  HIP_TRY(launch_fill_u32(c->d_stamp, 0u, c->N, c->stream));
+  c->nbr_tcap = (int32_t)pow2_at_least((uint64_t)(2 * (c->nnb + 64)));
+  c->nbr_use_lds = (4 * (int64_t)c->nbr_tcap * 8 <= 64 * 1024) ? 1 : 0;
+  if (!c->nbr_use_lds) {
+    const int64_t budget = (int64_t)1 << 30;
+    c->nbr_gslots = (int32_t)std::max<int64_t>(
+        1, std::min<int64_t>(c->node_cap, budget / ((int64_t)c->nbr_tcap * 8)));
+    ALLOC_TRY(c->d_nbr_gtable, (size_t)c->nbr_gslots * c->nbr_tcap);
+  }
+  c->beta_blocks = (int32_t)std::max<int64_t>(
+      1, std::min<int64_t>(256, (c->edge_cap + 4 * rpw * 4 - 1) / (4 * rpw * 4)));
+  ALLOC_TRY(c->d_beta_partials, (size_t)c->beta_blocks * kBetaWaves * 2 * c->K);
+  c->ppx_blocks = (int32_t)std::max<int64_t>(
+      1, std::min<int64_t>(2048, (c->H + 4 * rpw * 8 - 1) / (4 * rpw * 8)));
+  ALLOC_TRY(c->d_ppx_partials, (size_t)c->ppx_blocks * 4);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (o.verbose) {
+    // learner.cc:105-120, mcmc_sampler_stochastic.cc:84-91
+    const double link_ratio = (double)c->E / (((double)c->N * (double)(c->N - 1)) / 2.0);
+    fprintf(stderr, "Floating point precision: %dbit\n", (int)(c->fsize * 8));
+    fprintf(stderr, "N %d E %lld link ratio %g K %d\n", c->N, (long long)c->E, link_ratio, c->K);
+    fprintf(stderr, "minibatch size %lld epsilon %g alpha %g iterations %lld convergence %g\n",
+            (long long)c->m, o.epsilon, o.alpha, (long long)o.max_iteration,
+            o.convergence_threshold);
+    fprintf(stderr, "num_node_sample %lld\n", (long long)c->n);
+    fprintf(stderr, "a %g b %g c %g eta (%g,%g)\n", o.a, o.b, o.c, o.eta0, o.eta1);
+  }
+  return AMMSB_OK;
+}
+
+int set_beta_from_theta(ammsb_ctx* c, const double* theta) {
+  std::vector<double> beta(c->K);
+  for (int k = 0; k < c->K; ++k) {
+    // np::row_normalize + SelectColumn(1) (mcmc_sampler_stochastic.cc:59-63)
+    const double s = 0.0 + theta[2 * k] + theta[2 * k + 1];
+    beta[k] = theta[2 * k + 1] / s;
+  }
+  HIP_TRY(hipMemcpy(c->d_theta, theta, sizeof(double) * 2 * c->K, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_beta, beta.data(), sizeof(double) * c->K, hipMemcpyHostToDevice));
+  return AMMSB_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* ammsb_last_error(void) { return get_error(); }
+const char* ammsb_version(void) { return "ammsb-mi355x 0.1 (gfx950)"; }
+
+int ammsb_options_default(ammsb_options* o) {
+  if (!o) return AMMSB_ERR_ARG;
+  memset(o, 0, sizeof(*o));
+  o->alpha = 0.0;
+  o->eta0 = 1.0;
+  o->eta1 = 1.0;
+  o->epsilon = 0.0000001;
+  o->a = 0.0;
+  o->b = 1024;
+  o->c = 0.5;
+  o->K = 300;
+  o->mini_batch_size = 0;
+  o->num_node_sample = 0;
+  o->strategy = AMMSB_STRATIFIED_RANDOM_NODE;
+  o->sampler_max_source = 1;
+  o->max_iteration = 10000000;
+  o->interval = 0;
+  o->stats_print_interval = 0;
+  o->num_updates = 1000;
+  o->held_out_ratio = 0.0;
+  o->random_seed = 42;
+  o->convergence_threshold = 0.000000000001;
+  o->precision = AMMSB_FP64;
+  o->device = 0;
+  o->verbose = 0;
+  return AMMSB_OK;
+}
+
+int ammsb_create(const ammsb_options* opts, const ammsb_graph* g, const ammsb_split* split,
+                 ammsb_ctx** out) {
+  if (!opts || !out) {
+    set_error("ammsb_create: null argument");
+    return AMMSB_ERR_ARG;
+  }
+  *out = nullptr;
+  ammsb_ctx* c = nullptr;
+  try {
+    c = new ammsb_ctx();
+  } catch (...) {
+    set_error("ammsb_create: out of host memory");
+    return AMMSB_ERR_OOM;
+  }
+  int rc;
+  try {
+    rc = create_impl(opts, g, split, c);
+  } catch (const std::bad_alloc&) {
+    set_error("ammsb_create: out of host memory");
+    rc = AMMSB_ERR_OOM;
+  }
+  if (rc != AMMSB_OK) {
+    free_ctx(c);
+    return rc;
+  }
+  *out = c;
+  return AMMSB_OK;
+}
+
+int ammsb_destroy(ammsb_ctx* c) {
+  if (!c) return AMMSB_OK;
+  (void)hipSetDevice(c->opt.device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_ctx(c);
+  return AMMSB_OK;
+}
+
+int ammsb_get_info(const ammsb_ctx* c, ammsb_info* info) {
+  if (!c || !info) return AMMSB_ERR_ARG;
+  memset(info, 0, sizeof(*info));
+  info->N = c->N;
+  info->K = c->K;
+  info->num_edges = c->E;
+  info->ld = c->ld;
+  info->mini_batch_size = c->m;
+  info->num_node_sample = c->n;
+  info->interval = c->interval;
+  info->stats_print_interval = c->stats_interval;
+  info->max_minibatch_edges = c->edge_cap;
+  info->max_minibatch_nodes = c->node_cap;
+  info->held_out_count = c->H;
+  info->test_count = c->T;
+  info->step_count = c->step_count;
+  info->average_count = c->average_count;
+  info->alpha = c->opt.alpha;
+  info->a = c->opt.a;
+  info->b = c->opt.b;
+  info->c = c->opt.c;
+  info->epsilon = c->opt.epsilon;
+  info->eta0 = c->opt.eta0;
+  info->eta1 = c->opt.eta1;
+  info->precision = c->prec;
+  info->strategy = c->opt.strategy;
+  info->sampler_max_source = c->opt.sampler_max_source;
+  info->max_iteration = c->opt.max_iteration;
+  return AMMSB_OK;
+}
+
+int ammsb_init_state(ammsb_ctx* c, uint64_t seed) {
+  if (!c) return AMMSB_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->opt.device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const Key key = make_key(seed);
+  // theta = gamma(eta0, eta1, K, 2): shape eta0, scale eta1 for every entry
+  // (mcmc_sampler_stochastic.cc:54; random.cc:472-511 parameter order)
+  std::vector<double> theta(2 * c->K);
+  for (int k = 0; k < c->K; ++k)
+    for (int i = 0; i < 2; ++i)
+      theta[2 * k + i] =
+          std::exp(log_gamma_draw(key, c->opt.eta0, (uint32_t)(2 * k + i), 0, TAG_INIT_THETA)) *
+          c->opt.eta1;
+  int rc = set_beta_from_theta(c, theta.data());
+  if (rc != AMMSB_OK) return rc;
+  HIP_TRY(launch_init_phi(c->d_pi, c->ld, c->K, c->N, key, c->prec, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->state_ready = true;
+  return AMMSB_OK;
+}
+
+int ammsb_set_state(ammsb_ctx* c, const void* pi, const double* theta) {
+  if (!c || !pi || !theta) return AMMSB_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->opt.device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy2D(c->d_pi, c->ld * c->fsize, pi, (c->K + 1) * c->fsize, (c->K + 1) * c->fsize,
+                      c->N, hipMemcpyHostToDevice));
+  int rc = set_beta_from_theta(c, theta);
+  if (rc != AMMSB_OK) return rc;
+  c->state_ready = true;
+  return AMMSB_OK;
+}
+
+int ammsb_get_state(ammsb_ctx* c, void* pi, double* theta, double* beta) {
+  if (!c) return AMMSB_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->opt.device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (pi)
+    HIP_TRY(hipMemcpy2D(pi, (c->K + 1) * c->fsize, c->d_pi, c->ld * c->fsize,
+                        (c->K + 1) * c->fsize, c->N, hipMemcpyDeviceToHost));
+  if (theta) HIP_TRY(hipMemcpy(theta, c->d_theta, sizeof(double) * 2 * c->K, hipMemcpyDeviceToHost));
+  if (beta) HIP_TRY(hipMemcpy(beta, c->d_beta, sizeof(double) * c->K, hipMemcpyDeviceToHost));
+  return AMMSB_OK;
+}
+
+int ammsb_get_rows(ammsb_ctx* c, const int32_t* nodes, int64_t n, void* out) {
+  if (!c || (n > 0 && (!nodes || !out))) return AMMSB_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->opt.device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const size_t row = (c->K + 1) * c->fsize;
+  for (int64_t i = 0; i < n; ++i) {
+    if (nodes[i] < 0 || nodes[i] >= c->N) {
+      set_error("ammsb_get_rows: node out of range");
+      return AMMSB_ERR_ARG;
+    }
+    HIP_TRY(hipMemcpy((char*)out + i * row, (char*)c->d_pi + (size_t)nodes[i] * c->ld * c->fsize,
+                      row, hipMemcpyDeviceToHost));
+  }
+  return AMMSB_OK;
+}
+
+int ammsb_set_counters(ammsb_ctx* c, int64_t step_count, int64_t average_count) {
+  if (!c || step_count < 1 || average_count < 1) return AMMSB_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->opt.device));
+  c->step_count = step_count;
+  c->average_count = average_count;
+  HIP_TRY(launch_set_state_step(c->d_st, step_count, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AMMSB_OK;
+}
+
+int ammsb_step(ammsb_ctx* c, int64_t iters, int32_t sync) {
+  if (!c || iters < 0) return AMMSB_ERR_ARG;
+  if (!c->state_ready) {
+    set_error("ammsb_step: state not initialised (call ammsb_init_state or ammsb_set_state)");
+    return AMMSB_ERR_STATE;
+  }
+  HIP_TRY(hipSetDevice(c->opt.device));
+  for (int64_t i = 0; i < iters; ++i) {
+    PhaseScope ps(c, PH_OUTER);
+    int rc = enqueue_iteration(c);
+    if (rc != AMMSB_OK) return rc;
+    c->step_count++;
+  }
+  if (sync) return check_devstate(c);
+  return AMMSB_OK;
+}
+
+int ammsb_perplexity(ammsb_ctx* c, double* ppx) {
+  if (!c || !ppx) return AMMSB_ERR_ARG;
+  if (!c->state_ready) {
+    set_error("ammsb_perplexity: state not initialised");
+    return AMMSB_ERR_STATE;
+  }
+  HIP_TRY(hipSetDevice(c->opt.device));
+  return do_perplexity(c, ppx);
+}
+
+int ammsb_run(ammsb_ctx* c, int64_t max_steps, int32_t print, double* ppx_out,
+              int64_t* ppx_steps, int64_t cap, int64_t* n_ppx) {
+  if (!c) return AMMSB_ERR_ARG;
+  if (!c->state_ready) {
+    set_error("ammsb_run: state not initialised");
+    return AMMSB_ERR_STATE;
+  }
+  HIP_TRY(hipSetDevice(c->opt.device));
+  const auto t_start = std::chrono::system_clock::now();
+  int64_t done = 0, np = 0;
+  int rc = AMMSB_OK;
+  // mcmc_sampler_stochastic.cc:110-180
+  while (c->step_count < c->opt.max_iteration && !is_converged(c)) {
+    if (max_steps > 0 && done >= max_steps) break;
+    if ((c->step_count - 1) % c->interval == 0) {
+      double ppx = 0.0;
+      rc = do_perplexity(c, &ppx);
+      if (rc != AMMSB_OK) return rc;
+      rc = check_devstate(c);
+      if (rc != AMMSB_OK) return rc;
+      if (print) {
+        const auto t_ms = std::chrono::duration_cast<std::chrono::milliseconds>(
+                              std::chrono::system_clock::now() - t_start)
+                              .count();
+        printf("average_count is: %lld step count: %lld time: %.3f perplexity for hold out set: "
+               "%.12f\n",
+               (long long)c->average_count, (long long)c->step_count, t_ms / 1000.0, ppx);
+        fflush(stdout);
+      }
+      c->ppx_cb.push_back(ppx);
+      if (c->ppx_cb.size() > 10) c->ppx_cb.pop_front();
+      if (ppx_out && ppx_steps && np < cap) {
+        ppx_out[np] = ppx;
+        ppx_steps[np] = c->step_count;
+      }
+      ++np;
+    }
+    {
+      PhaseScope ps(c, PH_OUTER);
+      rc = enqueue_iteration(c);
+      if (rc != AMMSB_OK) return rc;
+    }
+    c->step_count++;
+    ++done;
+    if (print && c->step_count % c->stats_interval == 0) print_stats(c, stdout);
+  }
+  rc = check_devstate(c);
+  if (rc != AMMSB_OK) return rc;
+  if (print) print_stats(c, stdout);
+  if (n_ppx) *n_ppx = np;
+  return AMMSB_OK;
+}
+
+int ammsb_enable_timers(ammsb_ctx* c, int32_t on) {
+  if (!c) return AMMSB_ERR_ARG;
+  if (!on) flush_timers(c);
+  c->timers_on = on != 0;
+  return AMMSB_OK;
+}
+
+int ammsb_timers(ammsb_ctx* c, double* secs, int64_t* ticks) {
+  if (!c) return AMMSB_ERR_ARG;
+  int rc = flush_timers(c);
+  if (rc != AMMSB_OK) return rc;
+  for (int p = 0; p < PH_COUNT; ++p) {
+    if (secs) secs[p] = c->t_secs[p];
+    if (ticks) ticks[p] = c->t_ticks[p];
+  }
+  return AMMSB_OK;
+}
+
+int ammsb_update_phi(ammsb_ctx* c, const int32_t* nodes, int64_t n_nodes, const int32_t* nbrs,
+                     int64_t n_nb, const void* noise, double eps_t) {
+  if (!c || n_nodes < 0 || (n_nodes > 0 && (!nodes || !nbrs)) || n_nb < 1 || eps_t < 0.0)
+    return AMMSB_ERR_ARG;
+  if (n_nodes == 0) return AMMSB_OK;
+  if (n_nodes > INT32_MAX / 2 || n_nb > INT32_MAX / 2) return AMMSB_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->opt.device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  {
+    std::vector<int32_t> srt(nodes, nodes + n_nodes);
+    std::sort(srt.begin(), srt.end());
+    for (int64_t i = 0; i < n_nodes; ++i)
+      if (srt[i] < 0 || srt[i] >= c->N || (i && srt[i] == srt[i - 1])) {
+        set_error("ammsb_update_phi: nodes must be distinct and in [0, N)");
+        return AMMSB_ERR_ARG;
+      }
+    for (int64_t i = 0; i < n_nodes * n_nb; ++i)
+      if (nbrs[i] < 0 || nbrs[i] >= c->N) {
+        set_error("ammsb_update_phi: neighbour out of range");
+        return AMMSB_ERR_ARG;
+      }
+  }
+  int32_t *d_nodes = nullptr, *d_nbrs = nullptr;
+  void* d_noise = nullptr;
+  void* d_stage = nullptr;
+  int rc = AMMSB_OK;
+  auto cleanup = [&]() {
+    if (d_nodes) (void)hipFree(d_nodes);
+    if (d_nbrs) (void)hipFree(d_nbrs);
+    if (d_noise) (void)hipFree(d_noise);
+    if (d_stage) (void)hipFree(d_stage);
+  };
+  do {
+    if ((rc = dalloc(&d_nodes, n_nodes)) != AMMSB_OK) break;
+    if ((rc = dalloc(&d_nbrs, n_nodes * n_nb)) != AMMSB_OK) break;
+    char* tmp = nullptr;
+    if ((rc = dalloc(&tmp, n_nodes * c->ld * c->fsize)) != AMMSB_OK) break;
+    d_stage = tmp;
+    if (noise) {
+      char* nz = nullptr;
+      if ((rc = dalloc(&nz, n_nodes * c->K * c->fsize)) != AMMSB_OK) break;
+      d_noise = nz;
+    }
+  } while (0);
+  if (rc != AMMSB_OK) {
+    cleanup();
+    return rc;
+  }
+  hipError_t e = hipMemcpy(d_nodes, nodes, 4 * n_nodes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_nbrs, nbrs, 4 * n_nodes * n_nb, hipMemcpyHostToDevice);
+  if (e == hipSuccess && noise)
+    e = hipMemcpy(d_noise, noise, n_nodes * c->K * c->fsize, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_label_nbrs(dev_graph(c), d_nodes, (int32_t)n_nodes, d_nbrs, (int32_t)n_nb, c->stream);
+  if (e == hipSuccess) {
+    PhiArgs pa;
+    pa.pi = c->d_pi;
+    pa.ld = c->ld;
+    pa.K = c->K;
+    pa.N = c->N;
+    pa.nodes = d_nodes;
+    pa.count_dev = nullptr;
+    pa.count_host = (int32_t)n_nodes;
+    pa.nbrs = d_nbrs;
+    pa.nnb = (int32_t)n_nb;
+    pa.beta = c->d_beta;
+    pa.st = c->d_st;
+    pa.hp = c->hp;
+    pa.eps_override = eps_t;
+    pa.noise = d_noise;
+    pa.key = c->key;
+    pa.staging = d_stage;
+    e = launch_update_phi(pa, c->prec, (int32_t)n_nodes, c->stream);
+  }
+  if (e == hipSuccess) {
+    ScatterArgs sc;
+    sc.pi = c->d_pi;
+    sc.ld = c->ld;
+    sc.K = c->K;
+    sc.nodes = d_nodes;
+    sc.count_dev = nullptr;
+    sc.count_host = (int32_t)n_nodes;
+    sc.staging = d_stage;
+    e = launch_scatter(sc, c->prec, (int32_t)n_nodes, c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  cleanup();
+  if (e != hipSuccess) {
+    set_error(std::string("ammsb_update_phi: ") + hipGetErrorString(e));
+    return AMMSB_ERR_HIP;
+  }
+  return AMMSB_OK;
+}
+
+int ammsb_update_beta(ammsb_ctx* c, const int32_t* edges, const uint8_t* y, int64_t E,
+                      double scale, const double* noise, double eps_t) {
+  if (!c || E < 0 || (E > 0 && (!edges || !y)) || eps_t < 0.0) return AMMSB_ERR_ARG;
+  if (E > INT32_MAX / 2) return AMMSB_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->opt.device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int64_t i = 0; i < 2 * E; ++i)
+    if (edges[i] < 0 || edges[i] >= c->N) {
+      set_error("ammsb_update_beta: edge endpoint out of range");
+      return AMMSB_ERR_ARG;
+    }
+  int32_t* d_edges = nullptr;
+  uint8_t* d_y = nullptr;
+  double* d_noise = nullptr;
+  int rc = dalloc(&d_edges, 2 * E);
+  if (rc == AMMSB_OK) rc = dalloc(&d_y, E);
+  if (rc == AMMSB_OK && noise) rc = dalloc(&d_noise, 2 * c->K);
+  hipError_t e = hipSuccess;
+  if (rc == AMMSB_OK) {
+    if (E) e = hipMemcpy(d_edges, edges, 8 * E, hipMemcpyHostToDevice);
+    if (e == hipSuccess && E) e = hipMemcpy(d_y, y, E, hipMemcpyHostToDevice);
+    if (e == hipSuccess && noise)
+      e = hipMemcpy(d_noise, noise, 16 * c->K, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      BetaArgs ba;
+      ba.pi = c->d_pi;
+      ba.ld = c->ld;
+      ba.K = c->K;
+      ba.edges = d_edges;
+      ba.y = d_y;
+      ba.count_dev = nullptr;
+      ba.count_host = (int32_t)E;
+      ba.beta = c->d_beta;
+      ba.theta = c->d_theta;
+      ba.epsilon = c->hp.epsilon;
+      ba.partials = c->d_beta_partials;
+      ba.blocks = c->beta_blocks;
+      e = launch_beta(ba, c->prec, c->stream);
+    }
+    if (e == hipSuccess) {
+      ThetaArgs ta;
+      ta.theta = c->d_theta;
+      ta.beta = c->d_beta;
+      ta.K = c->K;
+      ta.partials = c->d_beta_partials;
+      ta.blocks = c->beta_blocks * kBetaWaves;
+      ta.st = c->d_st;
+      ta.hp = c->hp;
+      ta.eps_override = eps_t;
+      ta.scale_override = scale;
+      ta.noise = d_noise;
+      ta.key = c->key;
+      ta.bump_step = 0;
+      e = launch_theta(ta, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  if (d_edges) (void)hipFree(d_edges);
+  if (d_y) (void)hipFree(d_y);
+  if (d_noise) (void)hipFree(d_noise);
+  if (rc != AMMSB_OK) return rc;
+  if (e != hipSuccess) {
+    set_error(std::string("ammsb_update_beta: ") + hipGetErrorString(e));
+    return AMMSB_ERR_HIP;
+  }
+  return AMMSB_OK;
+}
+
+int ammsb_debug_sample(ammsb_ctx* c, int64_t step, int32_t* flag, double* scale, int32_t* edges,
+                       int64_t* n_edges, int32_t* nodes, int64_t* n_nodes, int32_t* nbrs) {
+  if (!c || step < 1) return AMMSB_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->opt.device));
+  const DevGraph g = dev_graph(c);
+  SamplerArgs sa;
+  sa.g = g;
+  sa.st = c->d_st;
+  sa.key = c->key;
+  sa.strategy = c->opt.strategy;
+  sa.m = c->m;
+  sa.max_source = c->opt.sampler_max_source;
+  sa.edge_cap = c->edge_cap;
+  sa.mb_edges = c->d_mb_edges;
+  sa.mb_y = c->d_mb_y;
+  sa.nodes = c->d_nodes;
+  sa.node_cap = c->node_cap;
+  sa.table = c->d_mbtable;
+  sa.table_cap = c->mbtable_cap;
+  sa.stamp = c->d_stamp;
+  sa.step_override = step;
+  HIP_TRY(launch_minibatch(sa, c->stream));
+  NeighborArgs na;
+  na.g = g;
+  na.st = c->d_st;
+  na.key = c->key;
+  na.nodes = c->d_nodes;
+  na.nnb = (int32_t)c->nnb;
+  na.nbrs = c->d_nbrs;
+  na.gtable = c->d_nbr_gtable;
+  na.tcap = c->nbr_tcap;
+  na.use_lds = c->nbr_use_lds;
+  na.step_override = step;
+  na.gtable_slots = c->nbr_gslots;
+  HIP_TRY(launch_neighbors(na, (int32_t)c->node_cap, c->stream));
+  DevState h;
+  int rc = read_devstate(c, &h);
+  if (rc != AMMSB_OK) return rc;
+  if (h.error) {
+    set_error("ammsb_debug_sample: device sampler error");
+    // clear the flag so later calls start clean
+    (void)launch_set_state_step(c->d_st, c->step_count, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    return AMMSB_ERR_INVARIANT;
+  }
+  if (flag) *flag = h.flag;
+  if (scale) *scale = h.scale;
+  if (n_edges) *n_edges = h.mb_count;
+  if (n_nodes) *n_nodes = h.node_count;
+  if (edges && h.mb_count)
+    HIP_TRY(hipMemcpy(edges, c->d_mb_edges, 8 * (size_t)h.mb_count, hipMemcpyDeviceToHost));
+  if (nodes && h.node_count)
+    HIP_TRY(hipMemcpy(nodes, c->d_nodes, 4 * (size_t)h.node_count, hipMemcpyDeviceToHost));
+  if (nbrs && h.node_count)
+    HIP_TRY(hipMemcpy(nbrs, c->d_nbrs, 4 * (size_t)h.node_count * c->nnb, hipMemcpyDeviceToHost));
+  return AMMSB_OK;
+}
+
+int ammsb_device_ptrs(ammsb_ctx* c, void** pi, void** theta, void** beta) {
+  if (!c) return AMMSB_ERR_ARG;
+  if (pi) *pi = c->d_pi;
+  if (theta) *theta = c->d_theta;
+  if (beta) *beta = c->d_beta;
+  return AMMSB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,190 @@
+// Launcher interface between the host driver (capi.hip) and the gfx950
+// kernels (kernels.hip).  Everything here is plain C++ (no templates cross
+// the boundary); the precision is a runtime switch that picks a template
+// instantiation inside kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "philox.h"
+
+namespace ammsb {
+
+// Device-resident per-context iteration state.  Every kernel of an
+// iteration reads `step` from here, so an iteration has no host-side
+// arguments that change between iterations (it can be replayed as a
+// captured graph).
+struct DevState {
+  int64_t step;        // Learner::step_count (1-based)
+  uint32_t epoch;      // bumped by every minibatch draw (node de-dup stamps)
+  int32_t flag;        // 0 = non-link minibatch, 1 = link minibatch
+  int32_t mb_count;    // minibatch edges
+  int32_t node_count;  // unique minibatch nodes
+  int32_t error;       // sampler error bits (ERR_*)
+  int32_t pad;
+  double scale;        // minibatch unbiasing scale (network.cc:341-342,385-390)
+  double ppx;          // last held-out perplexity
+};
+
+enum : int32_t {
+  ERR_NONLINK_ATTEMPTS = 1,   // non-link sampler exceeded its attempt budget
+  ERR_LINK_ROUNDS = 2,        // link sampler found only empty minibatches
+  ERR_NEIGHBOR_ATTEMPTS = 4,  // neighbour sampler exceeded its budget
+  ERR_MB_OVERFLOW = 8,        // minibatch larger than its buffer
+  ERR_BAD_STRATEGY = 16
+};
+
+// Read-only graph view on the device.
+struct DevGraph {
+  int32_t N;
+  int64_t E;                 // undirected edges
+  const int64_t* row_ptr;    // N+1
+  const int32_t* cols;       // 2E, rows ascending
+  const uint64_t* hkeys;     // held-out ∪ test hash (host_util.h layout)
+  const uint8_t* hvals;
+  uint64_t hmask;
+};
+
+// Hyper-parameters as the kernels see them (all double; each kernel casts
+// to its Float).
+struct DevHyper {
+  double alpha, epsilon, a, b, c, eta0, eta1;
+  double Nn;  // N / num_node_sample (mcmc_sampler_stochastic.cc:300)
+};
+
+struct SamplerArgs {
+  DevGraph g;
+  DevState* st;
+  Key key;
+  int32_t strategy;
+  int64_t m;             // mini_batch_size
+  int64_t max_source;
+  int64_t edge_cap;      // capacity of mb_edges
+  int32_t* mb_edges;     // [edge_cap][2]
+  uint8_t* mb_y;         // [edge_cap]
+  int32_t* nodes;        // [node_cap]
+  int64_t node_cap;
+  int2* table;           // non-link de-dup table (global), table_cap entries
+  int64_t table_cap;     // power of two >= 2*(m+1)
+  uint32_t* stamp;       // [N] node de-dup stamps
+  int64_t step_override; // <0: use st->step
+};
+
+struct NeighborArgs {
+  DevGraph g;
+  DevState* st;
+  Key key;
+  const int32_t* nodes;
+  int32_t nnb;            // n + 1
+  int32_t* nbrs;          // [node_cap][nnb], bit 31 = linked
+  int2* gtable;           // global fallback tables [node_cap][tcap] or null
+  int32_t tcap;           // per-node table capacity (power of two)
+  int32_t use_lds;
+  int64_t step_override;
+  int32_t gtable_slots;   // number of global tables (when !use_lds)
+};
+
+struct PhiArgs {
+  const void* pi;         // [N][ld]
+  int64_t ld;
+  int32_t K;
+  int32_t N;
+  const int32_t* nodes;
+  const int32_t* count_dev;  // node count on the device (or null)
+  int32_t count_host;        // used when count_dev == null
+  const int32_t* nbrs;       // [slot][nnb]
+  int32_t nnb;
+  const double* beta;        // [K]
+  const DevState* st;
+  DevHyper hp;
+  double eps_override;       // <0: derive eps_t from st->step
+  const void* noise;         // injected [slot][K] or null
+  Key key;
+  void* staging;             // [slot][ld]
+};
+
+struct ScatterArgs {
+  void* pi;
+  int64_t ld;
+  int32_t K;
+  const int32_t* nodes;
+  const int32_t* count_dev;
+  int32_t count_host;
+  const void* staging;
+};
+
+struct BetaArgs {
+  const void* pi;
+  int64_t ld;
+  int32_t K;
+  const int32_t* edges;     // [E][2]
+  const uint8_t* y;         // [E]
+  const int32_t* count_dev; // minibatch edge count on device (or null)
+  int32_t count_host;
+  const double* beta;
+  const double* theta;      // [K][2]
+  double epsilon;
+  double* partials;         // [blocks * kBetaWaves][2K]
+  int32_t blocks;
+};
+
+constexpr int kBetaWaves = 4;   // wavefronts per k_beta workgroup
+
+struct ThetaArgs {
+  double* theta;
+  double* beta;
+  int32_t K;
+  const double* partials;  // [rows][2K]
+  int32_t blocks;          // rows of partials
+  DevState* st;
+  DevHyper hp;
+  double eps_override;
+  double scale_override;   // <0: st->scale
+  const double* noise;     // injected [K][2] or null
+  Key key;
+  int32_t bump_step;       // 1: st->step += 1 at the end (end of an iteration)
+};
+
+struct PpxArgs {
+  const void* pi;
+  int64_t ld;
+  int32_t K;
+  const int32_t* edges;   // [H][2]
+  const uint8_t* labels;  // [H]
+  int64_t H;
+  double* R;              // running average per held-out edge
+  double avg_count;       // Learner::average_count
+  const double* beta;
+  double epsilon;
+  double* partials;       // [blocks][4]
+  int32_t blocks;
+  DevState* st;
+};
+
+// launchers (return hipError_t of the launch)
+hipError_t launch_minibatch(const SamplerArgs& a, hipStream_t s);
+hipError_t launch_neighbors(const NeighborArgs& a, int32_t grid_nodes, hipStream_t s);
+hipError_t launch_update_phi(const PhiArgs& a, int precision, int32_t grid_nodes,
+                             hipStream_t s);
+hipError_t launch_scatter(const ScatterArgs& a, int precision, int32_t grid_nodes,
+                          hipStream_t s);
+hipError_t launch_beta(const BetaArgs& a, int precision, hipStream_t s);
+hipError_t launch_theta(const ThetaArgs& a, hipStream_t s);
+hipError_t launch_perplexity(const PpxArgs& a, int precision, hipStream_t s);
+hipError_t launch_init_phi(void* pi, int64_t ld, int32_t K, int32_t N, Key key,
+                           int precision, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+hipError_t launch_fill_f64(double* p, double v, int64_t n, hipStream_t s);
+hipError_t launch_set_state_step(DevState* st, int64_t step, hipStream_t s);
+// Which (lanes-per-row, chunks) configuration serves K; false if K too big.
+bool row_config(int32_t K, int precision, int* lpr, int* ch);
+
+}  // namespace ammsb
+
+namespace ammsb {
+// Sets bit 31 of nbrs[i][j] when (nodes[i], nbr) is a link of the CSR
+// (injected-draw path of update_phi).
+hipError_t launch_label_nbrs(DevGraph g, const int32_t* nodes, int32_t n_nodes, int32_t* nbrs,
+                             int32_t nnb, hipStream_t s);
+}  // namespace ammsb

@@ -1,0 +1,205 @@
+"""Options of the sampler, mirroring ``mcmc::Options``.
+
+Reference: c++/src/mcmc/options.h:129-315 (Boost program_options with dotted
+``--mcmc.*`` flags, short aliases and an optional ``--config`` file) and the
+strategy names of c++/src/mcmc/types.h:11-60.  Defaults and the derived
+defaults (``a = b^-c``, ``alpha = 1/K``, options.h:257-262) are the same;
+unrecognised arguments are kept in ``remains`` like ``allow_unregistered``
+(options.h:244-255).  Values accept the reference's K/M/G suffixes
+(``parse_size_t``, c++/src/mcmc/options.cc:5-67).
+"""
+import ctypes
+import dataclasses
+import math
+
+from . import _lib
+
+STRATEGIES = {
+    "stratified-random-node": _lib.STRATIFIED_RANDOM_NODE,
+    "stratified-random-node-links": _lib.STRATIFIED_RANDOM_NODE_LINKS,
+    "stratified-random-node-nonlinks": _lib.STRATIFIED_RANDOM_NODE_NONLINKS,
+    "random-edge": _lib.RANDOM_EDGE,
+}
+
+
+def parse_size(text):
+    """Integer with optional k/K/m/M/g/G (powers of 1024) or kB/mB (powers of
+    1000) suffix, as options.cc:5-67."""
+    s = str(text).strip()
+    mult = 1
+    low = s.lower()
+    for suf, m in (("kb", 1000), ("mb", 1000 ** 2), ("gb", 1000 ** 3),
+                   ("k", 1024), ("m", 1024 ** 2), ("g", 1024 ** 3)):
+        if low.endswith(suf) and len(s) > len(suf):
+            mult = m
+            s = s[: -len(suf)]
+            break
+    return int(s, 0) * mult
+
+
+@dataclasses.dataclass
+class Options:
+    # --mcmc.* (options.h:139-189)
+    alpha: float = 0.0
+    eta0: float = 1.0
+    eta1: float = 1.0
+    epsilon: float = 0.0000001
+    a: float = 0.0
+    b: float = 1024.0
+    c: float = 0.5
+    K: int = 300
+    mini_batch_size: int = 0
+    num_node_sample: int = 0
+    strategy: str = "stratified-random-node"
+    sampler_max_source: int = 1
+    max_iteration: int = 10000000
+    interval: int = 0
+    stats_print_interval: int = 0
+    num_updates: int = 1000
+    held_out_ratio: float = 0.0
+    random_seed: int = 42
+    convergence_threshold: float = 0.000000000001
+    # input options (options.h:192-205)
+    input_filename: str = ""
+    input_class: str = "relativity"
+    input_contiguous: bool = False
+    input_compressed: bool = False
+    # this implementation
+    precision: str = "fp64"
+    device: int = 0
+    verbose: bool = False
+    remains: list = dataclasses.field(default_factory=list)
+
+    # flag name -> (field, converter); short aliases of options.h
+    _FLAGS = {
+        "mcmc.alpha": ("alpha", float), "mcmc.eta0": ("eta0", float),
+        "mcmc.eta1": ("eta1", float), "mcmc.epsilon": ("epsilon", float),
+        "mcmc.a": ("a", float), "mcmc.b": ("b", float), "mcmc.c": ("c", float),
+        "mcmc.K": ("K", parse_size), "K": ("K", parse_size),
+        "mcmc.mini-batch-size": ("mini_batch_size", parse_size), "m": ("mini_batch_size", parse_size),
+        "mcmc.num-node-sample": ("num_node_sample", parse_size), "n": ("num_node_sample", parse_size),
+        "mcmc.strategy": ("strategy", str),
+        "mcmc.sampler.max-source": ("sampler_max_source", parse_size),
+        "mcmc.max-iteration": ("max_iteration", parse_size), "x": ("max_iteration", parse_size),
+        "mcmc.interval": ("interval", parse_size), "i": ("interval", parse_size),
+        "mcmc.stats": ("stats_print_interval", parse_size), "I": ("stats_print_interval", parse_size),
+        "mcmc.num-updates": ("num_updates", parse_size),
+        "mcmc.held-out-ratio": ("held_out_ratio", float), "h": ("held_out_ratio", float),
+        "mcmc.seed": ("random_seed", int),
+        "mcmc.convergence": ("convergence_threshold", float),
+        "mcmc.input.file": ("input_filename", str), "f": ("input_filename", str),
+        "mcmc.input.class": ("input_class", str), "c": ("input_class", str),
+        "mcmc.precision": ("precision", str),
+        "mcmc.device": ("device", int),
+    }
+    _SWITCHES = {"mcmc.input.contiguous": "input_contiguous",
+                 "mcmc.input.compressed": "input_compressed", "verbose": "verbose"}
+
+    @classmethod
+    def parse(cls, argv):
+        """Parse ``--mcmc.K 32``, ``--mcmc.K=32``, ``-K 32``, ``--config F``."""
+        o = cls()
+        config = None
+        rest = []
+        i = 0
+        argv = list(argv)
+        while i < len(argv):
+            tok = argv[i]
+            if tok.startswith("--"):
+                name, eq, val = tok[2:].partition("=")
+            elif tok.startswith("-") and len(tok) >= 2 and not _is_number(tok):
+                name, eq, val = tok[1:2], "", tok[2:]
+                if val:
+                    eq = "="
+            else:
+                rest.append(tok)
+                i += 1
+                continue
+            if name == "config":
+                config = val if eq else argv[i + 1]
+                i += 1 if eq else 2
+                continue
+            if name in cls._SWITCHES:
+                setattr(o, cls._SWITCHES[name], True)
+                i += 1
+                continue
+            if name in cls._FLAGS:
+                field, conv = cls._FLAGS[name]
+                if not eq:
+                    if i + 1 >= len(argv):
+                        raise ValueError(f"option --{name} needs a value")
+                    val = argv[i + 1]
+                    i += 2
+                else:
+                    i += 1
+                o._set(field, conv, val)
+                continue
+            rest.append(tok)
+            i += 1
+        if config is not None:
+            o._read_config(config, cmdline=argv)
+        o.remains = rest
+        o._validate()
+        return o
+
+    def _set(self, field, conv, val):
+        if field == "strategy" and val not in STRATEGIES:
+            raise ValueError(f"Unknown strategy {val!r}")
+        setattr(self, field, conv(val))
+
+    def _read_config(self, path, cmdline):
+        # Boost parse_config_file: "name = value" lines, [section] prefixes;
+        # command-line values win (stored first, options.h:249-252)
+        given = {t[2:].partition("=")[0] for t in cmdline if t.startswith("--")}
+        section = ""
+        with open(path) as f:
+            for line in f:
+                line = line.split("#", 1)[0].strip()
+                if not line:
+                    continue
+                if line.startswith("[") and line.endswith("]"):
+                    section = line[1:-1].strip() + "."
+                    continue
+                key, _, val = line.partition("=")
+                key = section + key.strip()
+                val = val.strip()
+                if key in given:
+                    continue
+                if key in self._FLAGS:
+                    field, conv = self._FLAGS[key]
+                    self._set(field, conv, val)
+                elif key in self._SWITCHES:
+                    setattr(self, self._SWITCHES[key], val.lower() in ("1", "true", "yes", "on"))
+
+    def _validate(self):
+        if self.strategy not in STRATEGIES:
+            raise ValueError(f"Unknown strategy {self.strategy!r}")
+        if self.precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be fp64 or fp32")
+
+    def derived(self):
+        """(a, alpha) after the reference's derivation (options.h:257-262)."""
+        a = math.pow(self.b, -self.c) if self.a == 0.0 else self.a
+        alpha = 1.0 / self.K if self.alpha == 0.0 else self.alpha
+        return a, alpha
+
+    def to_c(self):
+        c = _lib.Options()
+        _lib.check(_lib.lib.ammsb_options_default(ctypes.byref(c)))
+        for f in ("alpha", "eta0", "eta1", "epsilon", "a", "b", "c", "K", "mini_batch_size",
+                  "num_node_sample", "sampler_max_source", "max_iteration", "interval",
+                  "stats_print_interval", "num_updates", "held_out_ratio", "random_seed",
+                  "convergence_threshold", "device"):
+            setattr(c, f, getattr(self, f))
+        c.strategy = STRATEGIES[self.strategy]
+        c.precision = _lib.FP64 if self.precision == "fp64" else _lib.FP32
+        c.verbose = 1 if self.verbose else 0
+        return c
+
+
+def _is_number(tok):
+    try:
+        float(tok)
+        return True
+    except ValueError:
+        return False

@@ -1,0 +1,238 @@
+"""Parity of the HIP path against the CPU oracle (GPU tests).
+
+Each test calls the device kernels through the C-ABI (libammsb.so) and the
+oracle (oracle/liboracle.so) on the same seeded inputs.  Tolerances:
+  * sampled indices (minibatch edges, flags, neighbours): bit-exact
+  * FP64 operators with injected draws: 1e-12 relative, with an absolute
+    floor of 1e-12 x the row's largest |value| (summation order differs)
+  * perplexity trajectory: within 1% of the oracle at every checkpoint
+"""
+import numpy as np
+import pytest
+
+import mcmc_for_ammsb_amd as M
+from tests.oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+def _state(N, K, seed, dtype=np.float64):
+    rng = np.random.default_rng(seed)
+    phi = rng.gamma(1.0, 1.0, size=(N, K))
+    s = phi.sum(1)
+    pi = np.concatenate([phi / s[:, None], s[:, None]], 1).astype(dtype)
+    theta = rng.gamma(2.0, 0.5, size=(K, 2))
+    return pi, theta
+
+
+def _close(gpu, ref, tol):
+    gpu = np.asarray(gpu, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    scale = np.max(np.abs(ref), axis=-1, keepdims=True) if ref.ndim > 1 else np.max(np.abs(ref))
+    err = np.abs(gpu - ref)
+    bound = tol * np.abs(ref) + tol * scale
+    return bool(np.all(err <= bound)), float(np.max(err / (np.abs(ref) + 1e-300)))
+
+
+GRAPHS = {}
+
+
+def graph(kind, N, K, deg=20.0, seed=3):
+    key = (kind, N, K, deg, seed)
+    if key not in GRAPHS:
+        if kind == "planted":
+            g = M.generate("planted", N=N, K=K, seed=seed)
+        else:
+            g = M.generate("community", N=N, K=K, mean_degree=deg, seed=seed)
+        GRAPHS[key] = (g, M.make_split(g, 0.1, 42))
+    return GRAPHS[key]
+
+
+def make_pair(kind, N, K, precision="fp64", n=32, m=None, strategy="stratified-random-node",
+              max_source=1, seed=42, deg=20.0, **hyp):
+    g, s = graph(kind, N, K, deg)
+    m = m if m is not None else max(1, N // 10)
+    opts = M.Options(K=K, num_node_sample=n, mini_batch_size=m, strategy=strategy,
+                     sampler_max_source=max_source, random_seed=seed, precision=precision, **hyp)
+    smp = M.MCMCSamplerStochastic(opts, g, s)
+    orc = Oracle(g, s, K, precision=precision, num_node_sample=n, alpha=hyp.get("alpha"),
+                 a=hyp.get("a"), b=hyp.get("b", 1024.0), c=hyp.get("c", 0.5),
+                 epsilon=hyp.get("epsilon", 1e-7), eta0=hyp.get("eta0", 1.0),
+                 eta1=hyp.get("eta1", 1.0))
+    return g, s, smp, orc
+
+
+@pytest.mark.parametrize("kind,N,K", [("planted", 2000, 32), ("community", 20000, 256),
+                                      ("community", 8000, 300), ("community", 4000, 7),
+                                      ("community", 3000, 1024)])
+def test_update_phi_fp64_injected(kind, N, K):
+    print(f"update_phi fp64 {kind} N={N} K={K}", flush=True)
+    g, s, smp, orc = make_pair(kind, N, K, n=24)
+    pi, theta = _state(N, K, 7)
+    smp.set_state(pi, theta)
+    orc.set_state(pi, theta)
+    rng = np.random.default_rng(11)
+    nodes = rng.choice(N, size=min(300, N // 4), replace=False).astype(np.int32)
+    nnb = 25
+    nbrs = np.stack([orc.sample_neighbors(int(v), nnb, 42, 5) & 0x7FFFFFFF for v in nodes])
+    nbrs[0, 3] = nodes[0]  # a self entry is skipped (mcmc_sampler_stochastic.cc:275-277)
+    noise = rng.standard_normal((len(nodes), K))
+    eps_t = 0.0123
+    smp.update_phi(nodes, nbrs, noise, eps_t)
+    orc.update_phi(nodes, nbrs, noise, eps_t)
+    gpi, _, _ = smp.get_state()
+    opi, _, _ = orc.get_state()
+    ok, rel = _close(gpi[nodes, :K], opi[nodes, :K], TOL64)
+    assert ok, rel
+    ok, rel = _close(gpi[nodes, K], opi[nodes, K], TOL64)
+    assert ok, rel
+    untouched = np.setdiff1d(np.arange(N), nodes)
+    assert np.array_equal(gpi[untouched], pi[untouched])
+
+
+@pytest.mark.parametrize("K", [32, 1024])
+def test_update_phi_fp32_injected(K):
+    print(f"update_phi fp32 K={K}", flush=True)
+    N = 4000
+    g, s, smp, orc = make_pair("community", N, K, precision="fp32", n=24)
+    pi, theta = _state(N, K, 8, np.float32)
+    smp.set_state(pi, theta)
+    orc.set_state(pi, theta)
+    rng = np.random.default_rng(12)
+    nodes = rng.choice(N, size=200, replace=False).astype(np.int32)
+    nbrs = np.stack([orc.sample_neighbors(int(v), 25, 42, 9) & 0x7FFFFFFF for v in nodes])
+    noise = rng.standard_normal((len(nodes), K)).astype(np.float32)
+    smp.update_phi(nodes, nbrs, noise, 0.01)
+    orc.update_phi(nodes, nbrs, noise, 0.01)
+    gpi, _, _ = smp.get_state()
+    opi, _, _ = orc.get_state()
+    # FP32 bulk arithmetic, different summation order: 1e-4 relative
+    ok, rel = _close(gpi[nodes], opi[nodes], 1e-4)
+    assert ok, rel
+
+
+@pytest.mark.parametrize("strategy,max_source", [("stratified-random-node-nonlinks", 1),
+                                                 ("stratified-random-node-links", 1),
+                                                 ("stratified-random-node-links", 8)])
+@pytest.mark.parametrize("K", [32, 256])
+def test_update_beta_fp64_injected(strategy, max_source, K):
+    print(f"update_beta fp64 {strategy} {max_source} K={K}", flush=True)
+    N = 6000
+    strat = {"stratified-random-node-nonlinks": 2, "stratified-random-node-links": 1}[strategy]
+    g, s, smp, orc = make_pair("community", N, K, n=16, m=600, max_source=max_source)
+    pi, theta = _state(N, K, 9)
+    smp.set_state(pi, theta)
+    orc.set_state(pi, theta)
+    mb = orc.sample_minibatch(strat, 600, max_source, 42, 3)
+    assert len(mb["y"]) > 0
+    noise = np.random.default_rng(3).standard_normal((K, 2))
+    smp.update_beta(mb["edges"], mb["y"], mb["scale"], noise, 0.02)
+    orc.update_beta(mb["edges"], mb["y"], mb["scale"], noise, 0.02)
+    _, gth, gbe = smp.get_state()
+    _, oth, obe = orc.get_state()
+    ok, rel = _close(gth, oth, TOL64)
+    assert ok, rel
+    ok, rel = _close(gbe, obe, TOL64)
+    assert ok, rel
+
+
+@pytest.mark.parametrize("K,prec", [(32, "fp64"), (256, "fp64"), (300, "fp64"), (64, "fp32")])
+def test_perplexity_running_average(K, prec):
+    print(f"perplexity K={K} {prec}", flush=True)
+    N = 5000
+    dt = np.float64 if prec == "fp64" else np.float32
+    g, s, smp, orc = make_pair("community", N, K, precision=prec)
+    tol = TOL64 if prec == "fp64" else 1e-5
+    for it in range(3):
+        pi, theta = _state(N, K, 20 + it, dt)
+        smp.set_state(pi, theta)
+        orc.set_state(pi, theta)
+        a = smp.perplexity()
+        b = orc.perplexity()
+        assert abs(a - b) <= tol * abs(b), (it, a, b)
+    assert smp.info["average_count"] == 4
+
+
+@pytest.mark.parametrize("strategy,max_source", [("stratified-random-node", 1),
+                                                 ("stratified-random-node-links", 16),
+                                                 ("stratified-random-node-nonlinks", 1)])
+def test_samplers_bit_exact(strategy, max_source):
+    print(f"samplers {strategy} {max_source}", flush=True)
+    N, K, n = 20000, 16, 40
+    strat = {"stratified-random-node": 0, "stratified-random-node-links": 1,
+             "stratified-random-node-nonlinks": 2}[strategy]
+    g, s, smp, orc = make_pair("community", N, K, n=n, m=1500, strategy=strategy,
+                               max_source=max_source, seed=77)
+    flags = set()
+    for step in range(1, 13):
+        d = smp.debug_sample(step)
+        r = orc.sample_minibatch(strat, 1500, max_source, 77, step)
+        flags.add(d["flag"])
+        assert d["flag"] == r["flag"], step
+        assert np.array_equal(d["edges"], r["edges"]), step
+        assert d["scale"] == pytest.approx(r["scale"], rel=1e-15)
+        ref_nodes = np.unique(r["edges"])
+        assert np.array_equal(np.sort(d["nodes"]), ref_nodes), step
+        for q in range(0, len(d["nodes"]), max(1, len(d["nodes"]) // 25)):
+            v = int(d["nodes"][q])
+            assert np.array_equal(d["nbrs"][q], orc.sample_neighbors(v, n + 1, 77, step)), (step, v)
+    if strat == 0:
+        assert flags == {0, 1}
+
+
+def test_trajectory_c1_within_one_percent():
+    """Config C1 (N=5000, K=32, n=32, a=0.01, b=1024, c=0.55) from the same
+    initial state and Philox seed: held-out perplexity within 1% at every
+    checkpoint of 1000 iterations."""
+    print("trajectory C1", flush=True)
+    N, K = 5000, 32
+    g, s, smp, orc = make_pair("planted", N, K, n=32, m=500, a=0.01, b=1024.0, c=0.55,
+                               interval=50, seed=42)
+    pi, theta = _state(N, K, 1)
+    smp.set_state(pi, theta)
+    orc.set_state(pi, theta)
+    gres = smp.run(max_steps=1000, print_log=False)
+    ores, _ = orc.run(1000, 0, 500, 1, 50, 42)
+    assert len(gres) == len(ores) == 20
+    for (gs, gp), (os_, op) in zip(gres, ores):
+        assert gs == os_
+        assert abs(gp - op) <= 0.01 * abs(op), (gs, gp, op)
+    print("checkpoints", [(a, round(b, 6)) for a, b in gres[::5]], flush=True)
+
+
+def test_init_state_properties():
+    N, K = 3000, 64
+    g, s = graph("community", N, K)
+    opts = M.Options(K=K, num_node_sample=10)
+    a = M.MCMCSamplerStochastic(opts, g, s)
+    b = M.MCMCSamplerStochastic(opts, g, s)
+    a.init(5)
+    b.init(5)
+    pa, ta, ba = a.get_state()
+    pb, tb, bb = b.get_state()
+    assert np.array_equal(pa, pb) and np.array_equal(ta, tb)
+    assert np.allclose(pa[:, :K].sum(1), 1.0, atol=1e-12)
+    assert np.all(pa[:, K] > 0) and np.all(ta > 0)
+    assert np.all((ba > 0) & (ba < 1))
+    assert np.allclose(ba, ta[:, 1] / ta.sum(1), rtol=1e-15)
+    # Gamma(1,1) rows: E[phi_sum] = K
+    assert abs(pa[:, K].mean() - K) < 0.05 * K
+
+
+def test_error_paths():
+    N, K = 2000, 16
+    g, s = graph("community", N, K)
+    smp = M.MCMCSamplerStochastic(M.Options(K=K, num_node_sample=8), g, s)
+    with pytest.raises(M.AmmsbError):
+        smp.step(1)  # before init
+    smp.init(1)
+    with pytest.raises(M.AmmsbError):
+        smp.update_phi([1, 1], np.zeros((2, 4), np.int32), None, 0.01)  # duplicate nodes
+    with pytest.raises(M.AmmsbError):
+        smp.update_phi([N + 5], np.zeros((1, 4), np.int32), None, 0.01)
+    with pytest.raises(M.AmmsbError):
+        M.MCMCSamplerStochastic(M.Options(K=K, num_node_sample=N), g, s)
+    smp.step(3)
+    assert smp.info["step_count"] == 4
