@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Benchmark of the SGRLD a-MMSB hot path on one MI355X.
+
+Metric: update_phi node*K updates/s -- nodes whose phi/pi rows are updated,
+times K, per second of whole SGRLD iterations (minibatch and neighbour
+sampling, update_phi, update_pi, update_beta, and the held-out perplexity
+at the reference's interval of 50 steps), with graph and state resident in
+HBM before the timed region starts.
+
+Default workload (one GPU): N=1,000,000, K=256, FP64, community graph with
+mean degree 50 (~25M edges), held-out ratio 0.1, alpha=1/K, eta=(1,1),
+epsilon=1e-7, a=0.01, b=1024, c=0.55, stratified random node minibatches of
+4,096 nodes (m=4095 non-link pairs; link minibatches gather sources until
+the next one would exceed m), 128 neighbour samples (129 drawn per node).
+
+The JSON line also carries
+  roofline      achieved HBM GB/s of the update_phi kernel (algorithmic
+                bytes per launch / its average launch time measured with HIP
+                events on the library's stream over the timed region)
+  cpu_baseline  the CPU oracle (C++ restatement of the reference loop) on
+                the host cores, on a bounded sample of the same workload
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (N, K, mean_degree, n, m, precision)
+    "c2": dict(N=1_000_000, K=256, deg=50.0, n=128, m=4095, precision="fp64",
+               desc="N=1M K=256 FP64 mean-degree-50 community graph, 4096-node minibatch, "
+                    "128 neighbour samples"),
+    "c3": dict(N=4_000_000, K=1024, deg=40.0, n=128, m=4095, precision="fp32",
+               desc="N=4M K=1024 FP32 precision mode, mean-degree-40 community graph, "
+                    "4096-node minibatch, 128 neighbour samples"),
+    "c3-fp64": dict(N=4_000_000, K=1024, deg=40.0, n=128, m=4095, precision="fp64",
+                    desc="N=4M K=1024 FP64, mean-degree-40 community graph, 4096-node "
+                         "minibatch, 128 neighbour samples"),
+    "c1": dict(N=5000, K=32, deg=None, n=32, m=500, precision="fp64",
+               desc="N=5000 K=32 FP64 planted graph, m=500, 32 neighbour samples"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(w, seed):
+    import mcmc_for_ammsb_amd as M
+    t = time.time()
+    if w["deg"] is None:
+        g = M.generate("planted", N=w["N"], K=w["K"], seed=seed)
+    else:
+        g = M.generate("community", N=w["N"], K=w["K"], mean_degree=w["deg"], seed=seed)
+    s = M.make_split(g, 0.1, 42)
+    log(f"graph N={g.N} E={g.num_edges} held-out={len(s.held_out_labels)} "
+        f"({time.time() - t:.1f}s)")
+    opts = M.Options(K=w["K"], num_node_sample=w["n"], mini_batch_size=w["m"],
+                     sampler_max_source=1024 if w["deg"] is not None else 1,
+                     a=0.01, b=1024.0, c=0.55, epsilon=1e-7, eta0=1.0, eta1=1.0,
+                     interval=50, random_seed=42, precision=w["precision"])
+    return g, s, opts
+
+
+def cpu_baseline(g, s, opts, pi, theta, start_step, budget_s, threads):
+    """The oracle loop on the host cores (bounded sample, no perplexity)."""
+    from tests.oracle_lib import Oracle
+    K = opts.K
+    orc = Oracle(g, s, K, precision=opts.precision, num_node_sample=opts.num_node_sample,
+                 a=opts.a, b=opts.b, c=opts.c, epsilon=opts.epsilon, eta0=opts.eta0,
+                 eta1=opts.eta1)
+    orc.set_state(pi, theta)
+    step = start_step if (start_step - 1) % 50 != 0 else start_step + 1
+    orc.set_counters(step, 2)
+    done, nodes, secs, phi_secs = 0, 0.0, 0.0, 0.0
+    t0 = time.time()
+    while time.time() - t0 < budget_s and done < 200:
+        ta = time.time()
+        _, ph = orc.run(1, 0, opts.mini_batch_size, opts.sampler_max_source,
+                        10 ** 9, 42, threads=threads)
+        secs += time.time() - ta
+        phi_secs += ph[3]
+        nodes += ph[5]
+        done += 1
+    return {
+        "value": nodes * K / secs,
+        "unit": "node*K/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{done} full iterations (samplers, update_phi+pi, update_beta) of the same "
+                  f"workload from the GPU run's state at step {step}, C++ oracle with "
+                  f"{threads} threads for the node loop",
+        "update_phi_node_k_per_s": nodes * K / phi_secs if phi_secs > 0 else None,
+        "seconds": secs,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU sample")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "update_phi_traffic.json"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        log("bench.py measures one GPU: this build has no multi-GPU path "
+            "(every box has a single MI355X); run with --gpus 1")
+        return 2
+    rank = int(os.environ.get("RANK", "0"))
+
+    import mcmc_for_ammsb_amd as M
+    w = WORKLOADS[args.workload]
+    g, s, opts = build_workload(w, args.seed)
+    smp = M.MCMCSamplerStochastic(opts, g, s)
+    inf = smp.info
+    smp.init(opts.random_seed)
+    K, nnb = inf["K"], inf["num_node_sample"] + 1
+    fsize = 8 if opts.precision == "fp64" else 4
+
+    smp.run(max_steps=args.warmup, print_log=False)
+    smp.enable_timers(True)
+    t_before = smp.timers()
+    n0, e0 = smp.totals()
+    t0 = time.perf_counter()
+    ckpt = smp.run(max_steps=args.steps, print_log=False)  # ends with a stream sync
+    t1 = time.perf_counter()
+    n1, e1 = smp.totals()
+    t_after = smp.timers()
+    smp.enable_timers(False)
+    elapsed = t1 - t0
+    nodes = n1 - n0
+    value = nodes * K / elapsed
+
+    ph = {k: (t_after[k][0] - t_before[k][0], t_after[k][1] - t_before[k][1]) for k in t_after}
+    phi_s, phi_n = ph["update_phi"]
+    avg_launch = phi_s / max(phi_n, 1)
+    nodes_per_launch = nodes / max(phi_n, 1)
+    # algorithmic bytes of one (node, k) update: n_nb neighbour rows + own row
+    # read + own row written; per node the n_nb neighbour ids and phi_sum
+    bytes_per_node = (nnb + 2) * K * fsize + nnb * 4 + 2 * fsize
+    bytes_per_launch = nodes_per_launch * bytes_per_node
+    achieved = bytes_per_launch / avg_launch / 1e9
+    traffic = None
+    if os.path.exists(args.traffic):
+        try:
+            with open(args.traffic) as f:
+                tr = json.load(f)
+            if tr.get("workload") == args.workload:
+                traffic = tr.get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    log("phase            total_s   ticks   per_tick_us")
+    for k, (sec, n) in ph.items():
+        log(f"{k:<16} {sec:9.4f} {n:7d} {1e6 * sec / max(n, 1):12.2f}")
+    log(f"checkpoints (step, perplexity): {ckpt}")
+
+    out = {
+        "metric": "update_phi node*K updates/s",
+        "value": value,
+        "unit": "node*K/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if opts.precision == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {w['desc']}", "N": g.N, "K": K,
+                   "edges": g.num_edges, "num_node_sample": inf["num_node_sample"],
+                   "mini_batch_size": inf["mini_batch_size"],
+                   "sampler_max_source": opts.sampler_max_source, "perplexity_interval": 50,
+                   "held_out": inf["held_out_count"], "nodes_per_step": nodes / args.steps,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_update_phi", "avg_launch_us": 1e6 * avg_launch,
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "nodes_per_launch": nodes_per_launch},
+        "update_phi_kernel_node_k_per_s": nodes * K / phi_s if phi_s > 0 else None,
+        "phase_us_per_step": {k: 1e6 * sec / args.steps for k, (sec, n) in ph.items()},
+        "perplexity_last": ckpt[-1][1] if ckpt else None,
+        "cpu_baseline": None,
+    }
+    if rank == 0 and not args.no_cpu_baseline:
+        pi, theta, _ = smp.get_state()
+        step = smp.info["step_count"]
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(g, s, opts, pi, theta, step, args.cpu_budget, threads)
+        del pi
+    smp.close()
+    if rank == 0:
+        print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

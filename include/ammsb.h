@@ -270,6 +270,11 @@ int ammsb_debug_sample(ammsb_ctx *ctx, int64_t step, int32_t *flag,
                        double *scale, int32_t *edges, int64_t *n_edges,
                        int32_t *nodes, int64_t *n_nodes, int32_t *nbrs);
 
+/* Totals over every iteration run so far (synchronises the stream): nodes
+ * whose phi/pi rows were updated and minibatch edges used by update_beta. */
+int ammsb_get_totals(ammsb_ctx *ctx, int64_t *nodes_updated,
+                     int64_t *minibatch_edges);
+
 /* Device pointers of the resident state for interop (torch DLPack-free
  * views): pi ([N][ld] of the precision type), theta ([K][2] double),
  * beta ([K] double). */

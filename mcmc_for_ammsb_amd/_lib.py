@@ -95,7 +95,7 @@ class AmmsbError(RuntimeError):
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
-            f"{LIB_PATH} is missing: build it with `python -m mcmc_for_ammsb_amd.build` "
+            f"{LIB_PATH} is missing: build it with `python mcmc_for_ammsb_amd/build.py` "
             "(the sampler has no CPU fallback)")
     lib = ctypes.CDLL(LIB_PATH)
     sig = {
@@ -127,6 +127,7 @@ def _load():
         "ammsb_update_beta": (c_i32, [c_vp, P_i32, P_u8, c_i64, c_dbl, P_dbl, c_dbl]),
         "ammsb_debug_sample": (c_i32, [c_vp, c_i64, P_i32, P_dbl, P_i32, P_i64, P_i32, P_i64,
                                        P_i32]),
+        "ammsb_get_totals": (c_i32, [c_vp, P_i64, P_i64]),
         "ammsb_device_ptrs": (c_i32, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp),
                                       ctypes.POINTER(c_vp)]),
     }

@@ -2,7 +2,8 @@
 
 Every HIP/C++ source under csrc/ is compiled to an object in parallel and
 linked into mcmc_for_ammsb_amd/libammsb.so, which the Python front end loads
-with ctypes.  Run as ``python -m mcmc_for_ammsb_amd.build``.
+with ctypes.  Run as ``python mcmc_for_ammsb_amd/build.py`` (a script, so a
+stale library never has to be imported to rebuild it).
 """
 import concurrent.futures
 import os

@@ -1107,6 +1107,17 @@ int ammsb_debug_sample(ammsb_ctx* c, int64_t step, int32_t* flag, double* scale,
   return AMMSB_OK;
 }
 
+int ammsb_get_totals(ammsb_ctx* c, int64_t* nodes_updated, int64_t* minibatch_edges) {
+  if (!c) return AMMSB_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->opt.device));
+  DevState h;
+  int rc = read_devstate(c, &h);
+  if (rc != AMMSB_OK) return rc;
+  if (nodes_updated) *nodes_updated = h.nodes_total;
+  if (minibatch_edges) *minibatch_edges = h.edges_total;
+  return AMMSB_OK;
+}
+
 int ammsb_device_ptrs(ammsb_ctx* c, void** pi, void** theta, void** beta) {
   if (!c) return AMMSB_ERR_ARG;
   if (pi) *pi = c->d_pi;

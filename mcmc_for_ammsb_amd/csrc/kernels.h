@@ -25,6 +25,8 @@ struct DevState {
   int32_t pad;
   double scale;        // minibatch unbiasing scale (network.cc:341-342,385-390)
   double ppx;          // last held-out perplexity
+  int64_t nodes_total; // sum of node_count over all minibatch draws
+  int64_t edges_total; // sum of mb_count over all minibatch draws
 };
 
 enum : int32_t {

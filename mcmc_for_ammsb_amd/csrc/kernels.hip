@@ -324,6 +324,10 @@ __global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
     st->node_count = min(s_nodes, (int)a.node_cap);
     st->scale = scale;
     st->error |= err;
+    if (a.step_override < 0) {
+      st->nodes_total += min(s_nodes, (int)a.node_cap);
+      st->edges_total += c;
+    }
   }
 }
 

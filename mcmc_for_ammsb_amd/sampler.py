@@ -183,6 +183,12 @@ class MCMCSamplerStochastic:
                                          _lib.ptr(ticks, ctypes.c_int64)))
         return {p: (float(s), int(t)) for p, s, t in zip(PHASES, secs, ticks)}
 
+    def totals(self):
+        """(nodes updated, minibatch edges) summed over all iterations."""
+        n, e = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(_lib.lib.ammsb_get_totals(self._h, ctypes.byref(n), ctypes.byref(e)))
+        return n.value, e.value
+
     def device_ptrs(self):
         pi, th, be = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
         _lib.check(_lib.lib.ammsb_device_ptrs(self._h, ctypes.byref(pi), ctypes.byref(th), ctypes.byref(be)))

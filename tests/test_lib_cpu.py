@@ -1,0 +1,120 @@
+"""Host-side checks of libammsb.so (no GPU): exported symbols, defaults,
+options parsing, synthetic generators, CSR construction and the split."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import mcmc_for_ammsb_amd as M
+from mcmc_for_ammsb_amd import _lib
+from mcmc_for_ammsb_amd.options import Options, parse_size
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "ammsb.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ammsb_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = header_functions()
+    assert len(names) >= 25
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_options_default_matches_reference():
+    # c++/src/mcmc/options.h:139-189
+    o = _lib.Options()
+    assert _lib.lib.ammsb_options_default(ctypes.byref(o)) == 0
+    assert (o.alpha, o.eta0, o.eta1, o.epsilon) == (0.0, 1.0, 1.0, 1e-7)
+    assert (o.a, o.b, o.c, o.K) == (0.0, 1024.0, 0.5, 300)
+    assert (o.mini_batch_size, o.num_node_sample, o.strategy, o.sampler_max_source) == (0, 0, 0, 1)
+    assert (o.max_iteration, o.interval, o.stats_print_interval, o.num_updates) == (10000000, 0, 0, 1000)
+    assert (o.held_out_ratio, o.random_seed, o.convergence_threshold) == (0.0, 42, 1e-12)
+
+
+def test_options_parse_flags_aliases_config(tmp_path):
+    o = Options.parse(["--mcmc.K", "32", "-m", "500", "--mcmc.num-node-sample=32", "--mcmc.a", "0.01",
+                       "--mcmc.c=0.55", "--mcmc.strategy", "stratified-random-node-links",
+                       "--mcmc.sampler.max-source", "8", "-x", "1k", "--unknown", "7", "pos"])
+    assert (o.K, o.mini_batch_size, o.num_node_sample, o.a, o.c) == (32, 500, 32, 0.01, 0.55)
+    assert o.strategy == "stratified-random-node-links" and o.sampler_max_source == 8
+    assert o.max_iteration == 1024
+    assert o.remains == ["--unknown", "7", "pos"]
+    cfg = tmp_path / "c.cfg"
+    cfg.write_text("[mcmc]\nK = 16\nepsilon = 1e-6\ninterval=10\n")
+    o = Options.parse(["--config", str(cfg), "--mcmc.K", "64"])
+    assert o.K == 64 and o.epsilon == 1e-6 and o.interval == 10  # command line wins
+    a, alpha = Options(K=64, b=1024.0, c=0.5).derived()
+    assert a == pytest.approx(1024 ** -0.5) and alpha == 1 / 64
+    with pytest.raises(ValueError):
+        Options.parse(["--mcmc.strategy", "nope"])
+    assert parse_size("2M") == 2 * 1024 ** 2 and parse_size("3kB") == 3000
+
+
+def _check_csr(g):
+    rp, c = g.row_ptr, g.cols
+    assert rp[0] == 0 and rp[-1] == len(c) and len(rp) == g.N + 1
+    src = np.repeat(np.arange(g.N), np.diff(rp))
+    assert np.all(src != c)
+    for i in range(0, g.N, max(1, g.N // 200)):
+        row = c[rp[i]:rp[i + 1]]
+        assert np.all(np.diff(row) > 0)
+    # symmetric: multiset of (src, dst) equals (dst, src)
+    a = np.sort(src.astype(np.int64) * g.N + c)
+    b = np.sort(c.astype(np.int64) * g.N + src)
+    assert np.array_equal(a, b)
+
+
+def test_planted_generator():
+    g = M.generate("planted", N=2000, K=16, seed=3)
+    _check_csr(g)
+    g2 = M.generate("planted", N=2000, K=16, seed=3)
+    assert np.array_equal(g.cols, g2.cols)
+    # expected edges ~ N^2/2 * E[sum_k pi_ak pi_bk beta_k] with Dir(0.05): ~ 2e6 * 0.05/16*... > 0
+    assert g.num_edges > 500
+
+
+def test_community_generator_mean_degree():
+    g = M.generate("community", N=50000, K=64, mean_degree=30.0, seed=1)
+    _check_csr(g)
+    assert abs(2 * g.num_edges / g.N - 30.0) < 1.5
+
+
+def test_split_semantics():
+    g = M.generate("community", N=20000, K=32, mean_degree=20.0, seed=2)
+    s = M.make_split(g, 0.1, 42)
+    p = int(0.1 * g.num_edges) // 2  # network.cc:72,504,562
+    assert len(s.held_out_labels) == 2 * p and len(s.test_labels) == 2 * p
+    assert s.held_out_labels.sum() == p and s.test_labels.sum() == p
+    ho = {tuple(e) for e in s.held_out_edges}
+    te = {tuple(e) for e in s.test_edges}
+    assert len(ho) == 2 * p and len(te) == 2 * p and not (ho & te)
+    for (a, b), y in zip(s.held_out_edges, s.held_out_labels):
+        assert a < b and g.linked(a, b) == bool(y)
+    for (a, b), y in zip(s.test_edges, s.test_labels):
+        assert a < b and g.linked(a, b) == bool(y)
+    s2 = M.make_split(g, 0.1, 42)
+    assert np.array_equal(s.held_out_edges, s2.held_out_edges)
+    with pytest.raises(M.AmmsbError):
+        M.make_split(M.generate("community", N=100, K=4, mean_degree=2.0, seed=1), 3.0, 1)
+
+
+def test_graph_from_edges_and_file(tmp_path):
+    f = tmp_path / "g.txt"
+    f.write_text("# comment\n10 20\n20 10\n10 30\n30 30\n40 10\n")
+    g = M.Graph.from_edge_list_file(str(f))
+    assert g.N == 4 and g.num_edges == 3
+    _check_csr(g)
+    assert g.linked(0, 1) and g.linked(0, 2) and g.linked(0, 3) and not g.linked(1, 2)
+
+
+def test_sampler_requires_graph_type():
+    with pytest.raises(TypeError):
+        M.MCMCSamplerStochastic(Options(K=4), "not a graph")
