@@ -113,6 +113,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--phi-waves", type=int, default=0, help="update_phi wavefronts per node")
+    ap.add_argument("--phi-unroll", type=int, default=0, help="update_phi rows in flight")
     ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "update_phi_traffic.json"))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -125,6 +127,10 @@ def main():
     w = WORKLOADS[args.workload]
     g, s, opts = build_workload(w, args.seed)
     smp = M.MCMCSamplerStochastic(opts, g, s)
+    if args.phi_waves or args.phi_unroll:
+        cur = smp.get_tuning()
+        smp.set_tuning(phi_waves=args.phi_waves or cur["phi_waves"],
+                       phi_unroll=args.phi_unroll or cur["phi_unroll"])
     inf = smp.info
     smp.init(opts.random_seed)
     K, nnb = inf["K"], inf["num_node_sample"] + 1
@@ -191,7 +197,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_update_phi", "avg_launch_us": 1e6 * avg_launch,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "nodes_per_launch": nodes_per_launch},
+                     "nodes_per_launch": nodes_per_launch, "launch_shape": smp.get_tuning()},
         "update_phi_kernel_node_k_per_s": nodes * K / phi_s if phi_s > 0 else None,
         "phase_us_per_step": {k: 1e6 * sec / args.steps for k, (sec, n) in ph.items()},
         "perplexity_last": ckpt[-1][1] if ckpt else None,

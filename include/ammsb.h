@@ -275,6 +275,15 @@ int ammsb_debug_sample(ammsb_ctx *ctx, int64_t step, int32_t *flag,
 int ammsb_get_totals(ammsb_ctx *ctx, int64_t *nodes_updated,
                      int64_t *minibatch_edges);
 
+/* Launch-shape tuning (no counterpart in the reference, which runs on CPU):
+ *   "phi_waves"  wavefronts per minibatch node in update_phi (2, 4, 8)
+ *   "phi_unroll" neighbour rows in flight per row group (1, 2, 4)
+ * Set phi_waves first; the (waves, unroll) pair is validated when
+ * phi_unroll is set.  Results do not depend on the shape beyond summation
+ * order.  Defaults (4, 2). */
+int ammsb_set_tuning(ammsb_ctx *ctx, const char *name, int64_t value);
+int ammsb_get_tuning(ammsb_ctx *ctx, const char *name, int64_t *value);
+
 /* Device pointers of the resident state for interop (torch DLPack-free
  * views): pi ([N][ld] of the precision type), theta ([K][2] double),
  * beta ([K] double). */

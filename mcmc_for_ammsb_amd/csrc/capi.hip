@@ -86,14 +86,14 @@ struct ammsb_ctx {
   double* d_beta = nullptr;
   DevState* d_st = nullptr;
   // iteration scratch
-  int32_t* d_mb_edges = nullptr;
-  uint8_t* d_mb_y = nullptr;
-  int32_t* d_nodes = nullptr;
-  int32_t* d_nbrs = nullptr;
+  Ring ring{};              // sample-ahead ring (device pointers)
+  int64_t ring_base = 1;    // host mirror of DevState::ring_base
+  bool ring_valid = false;
   char* d_staging = nullptr;  // [node_cap][ld]
   int2* d_mbtable = nullptr;
   int64_t mbtable_cap = 0;
-  uint32_t* d_stamp = nullptr;
+  int32_t mbtable_in_lds = 0;
+  int32_t phi_waves = kPhiWavesDefault, phi_unroll = kPhiUnrollDefault;
   int2* d_nbr_gtable = nullptr;
   int32_t nbr_tcap = 0, nbr_use_lds = 1, nbr_gslots = 0;
   double* d_beta_partials = nullptr;
@@ -210,8 +210,9 @@ struct PhaseScope {
   }
 };
 
-// ---- one SGRLD iteration on the device (mcmc_sampler_stochastic.cc:131-174)
-int enqueue_iteration(ammsb_ctx* c) {
+// ---- sample-ahead: draw `fills` iterations starting at step t0 (t0 < 0: the
+// device's step_count) into ring slots 0..fills-1
+int fill_ring(ammsb_ctx* c, int64_t t0, int32_t fills) {
   const DevGraph g = dev_graph(c);
   {
     PhaseScope ps(c, PH_MINIBATCH);
@@ -222,16 +223,12 @@ int enqueue_iteration(ammsb_ctx* c) {
     sa.strategy = c->opt.strategy;
     sa.m = c->m;
     sa.max_source = c->opt.sampler_max_source;
-    sa.edge_cap = c->edge_cap;
-    sa.mb_edges = c->d_mb_edges;
-    sa.mb_y = c->d_mb_y;
-    sa.nodes = c->d_nodes;
-    sa.node_cap = c->node_cap;
+    sa.ring = c->ring;
+    sa.t0 = t0;
     sa.table = c->d_mbtable;
     sa.table_cap = c->mbtable_cap;
-    sa.stamp = c->d_stamp;
-    sa.step_override = -1;
-    HIP_TRY(launch_minibatch(sa, c->stream));
+    sa.table_in_lds = c->mbtable_in_lds;
+    HIP_TRY(launch_minibatch(sa, fills, c->stream));
   }
   {
     PhaseScope ps(c, PH_NEIGHBORS);
@@ -239,15 +236,25 @@ int enqueue_iteration(ammsb_ctx* c) {
     na.g = g;
     na.st = c->d_st;
     na.key = c->key;
-    na.nodes = c->d_nodes;
-    na.nnb = (int32_t)c->nnb;
-    na.nbrs = c->d_nbrs;
+    na.ring = c->ring;
     na.gtable = c->d_nbr_gtable;
     na.tcap = c->nbr_tcap;
     na.use_lds = c->nbr_use_lds;
-    na.step_override = -1;
     na.gtable_slots = c->nbr_gslots;
-    HIP_TRY(launch_neighbors(na, (int32_t)c->node_cap, c->stream));
+    HIP_TRY(launch_neighbors(na, fills, c->stream));
+  }
+  return AMMSB_OK;
+}
+
+// ---- one SGRLD iteration on the device (mcmc_sampler_stochastic.cc:131-174)
+// The minibatch and neighbour draws come from the ring, refilled every
+// ring.slots iterations.
+int enqueue_iteration(ammsb_ctx* c) {
+  if (!c->ring_valid || c->step_count < c->ring_base || c->step_count >= c->ring_base + c->ring.slots) {
+    int rc = fill_ring(c, c->step_count, c->ring.slots);
+    if (rc != AMMSB_OK) return rc;
+    c->ring_base = c->step_count;
+    c->ring_valid = true;
   }
   {
     PhaseScope ps(c, PH_UPDATE_PHI);
@@ -256,10 +263,11 @@ int enqueue_iteration(ammsb_ctx* c) {
     pa.ld = c->ld;
     pa.K = c->K;
     pa.N = c->N;
-    pa.nodes = c->d_nodes;
-    pa.count_dev = &c->d_st->node_count;
+    pa.use_ring = 1;
+    pa.ring = c->ring;
+    pa.nodes = nullptr;
     pa.count_host = 0;
-    pa.nbrs = c->d_nbrs;
+    pa.nbrs = nullptr;
     pa.nnb = (int32_t)c->nnb;
     pa.beta = c->d_beta;
     pa.st = c->d_st;
@@ -268,7 +276,8 @@ int enqueue_iteration(ammsb_ctx* c) {
     pa.noise = nullptr;
     pa.key = c->key;
     pa.staging = c->d_staging;
-    HIP_TRY(launch_update_phi(pa, c->prec, (int32_t)c->node_cap, c->stream));
+    HIP_TRY(launch_update_phi(pa, c->prec, (int32_t)c->node_cap, c->stream, c->phi_waves,
+                              c->phi_unroll));
   }
   {
     PhaseScope ps(c, PH_UPDATE_PI);
@@ -276,8 +285,10 @@ int enqueue_iteration(ammsb_ctx* c) {
     sc.pi = c->d_pi;
     sc.ld = c->ld;
     sc.K = c->K;
-    sc.nodes = c->d_nodes;
-    sc.count_dev = &c->d_st->node_count;
+    sc.use_ring = 1;
+    sc.ring = c->ring;
+    sc.st = c->d_st;
+    sc.nodes = nullptr;
     sc.count_host = 0;
     sc.staging = c->d_staging;
     HIP_TRY(launch_scatter(sc, c->prec, (int32_t)c->node_cap, c->stream));
@@ -288,9 +299,11 @@ int enqueue_iteration(ammsb_ctx* c) {
     ba.pi = c->d_pi;
     ba.ld = c->ld;
     ba.K = c->K;
-    ba.edges = c->d_mb_edges;
-    ba.y = c->d_mb_y;
-    ba.count_dev = &c->d_st->mb_count;
+    ba.use_ring = 1;
+    ba.ring = c->ring;
+    ba.st = c->d_st;
+    ba.edges = nullptr;
+    ba.y = nullptr;
     ba.count_host = 0;
     ba.beta = c->d_beta;
     ba.theta = c->d_theta;
@@ -308,6 +321,7 @@ int enqueue_iteration(ammsb_ctx* c) {
     ta.hp = c->hp;
     ta.eps_override = -1.0;
     ta.scale_override = -1.0;
+    ta.meta = c->ring.meta;
     ta.noise = nullptr;
     ta.key = c->key;
     ta.bump_step = 1;
@@ -397,8 +411,9 @@ void free_ctx(ammsb_ctx* c) {
   if (!c) return;
   void* ptrs[] = {c->d_row_ptr, c->d_cols,    c->d_hkeys,    c->d_hvals,     c->d_ho_edges,
                   c->d_ho_labels, c->d_R,     c->d_pi,       c->d_theta,     c->d_beta,
-                  c->d_st,      c->d_mb_edges, c->d_mb_y,    c->d_nodes,     c->d_nbrs,
-                  c->d_staging, c->d_mbtable, c->d_stamp,    c->d_nbr_gtable, c->d_beta_partials,
+                  c->d_st,      c->ring.edges, c->ring.y,    c->ring.nodes,  c->ring.nbrs,
+                  c->d_staging, c->d_mbtable, c->ring.stamp, c->d_nbr_gtable, c->d_beta_partials,
+                  c->ring.meta,
                   c->d_ppx_partials};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -564,15 +579,44 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
   // ---- iteration scratch (sizes: network.cc:172-201)
   c->edge_cap = c->m + c->max_degree + 1;
   c->node_cap = std::min<int64_t>(c->N, c->edge_cap + std::min<int64_t>(o.sampler_max_source, c->N) + 1);
-  ALLOC_TRY(c->d_mb_edges, (size_t)(2 * c->edge_cap));
-  ALLOC_TRY(c->d_mb_y, (size_t)c->edge_cap);
-  ALLOC_TRY(c->d_nodes, (size_t)c->node_cap);
-  ALLOC_TRY(c->d_nbrs, (size_t)(c->node_cap * c->nnb));
+  {
+    // sample-ahead ring: up to 16 iterations, at most ~1 GiB
+    const int64_t slot_bytes = c->edge_cap * 9 + c->node_cap * 4 + c->node_cap * c->nnb * 4 +
+                               (int64_t)c->N * 4 + 64;
+    const int64_t slots = std::max<int64_t>(1, std::min<int64_t>(16, ((int64_t)1 << 30) / slot_bytes));
+    Ring& R = c->ring;
+    R.slots = (int32_t)slots;
+    R.edge_cap = c->edge_cap;
+    R.node_cap = c->node_cap;
+    R.nnb = (int32_t)c->nnb;
+    R.N = c->N;
+    ALLOC_TRY(R.edges, (size_t)(slots * 2 * c->edge_cap));
+    ALLOC_TRY(R.y, (size_t)(slots * c->edge_cap));
+    ALLOC_TRY(R.nodes, (size_t)(slots * c->node_cap));
+    ALLOC_TRY(R.nbrs, (size_t)(slots * c->node_cap * c->nnb));
+    ALLOC_TRY(R.meta, (size_t)slots);
+    ALLOC_TRY(R.stamp, (size_t)(slots * c->N));
+    HIP_TRY(launch_fill_u32(R.stamp, 0u, slots * c->N, c->stream));
+    HIP_TRY(hipMemsetAsync(R.meta, 0, sizeof(IterMeta) * slots, c->stream));
+  }
   ALLOC_TRY(c->d_staging, (size_t)(c->node_cap * c->ld * c->fsize));
-  c->mbtable_cap = (int64_t)pow2_at_least((uint64_t)(2 * (c->m + 1024)));
-  ALLOC_TRY(c->d_mbtable, (size_t)c->mbtable_cap);
-  ALLOC_TRY(c->d_stamp, (size_t)c->N);
-  HIP_TRY(launch_fill_u32(c->d_stamp, 0u, c->N, c->stream));
+  {
+    // non-link de-dup table: in LDS (load <= 0.8) when it fits next to the
+    // sampler's ~45 KB of static LDS, else in HBM (load <= 0.5)
+    int lds_max = 0;
+    HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, o.device));
+    const int64_t entries = c->m + 1024;
+    const int64_t lds_cap = (int64_t)pow2_at_least((uint64_t)(entries + entries / 4 + 1));
+    const int64_t static_lds = kMinibatchStaticLds;
+    if (lds_cap * (int64_t)sizeof(int2) + static_lds <= std::min<int64_t>(lds_max, 160 * 1024)) {
+      c->mbtable_cap = lds_cap;
+      c->mbtable_in_lds = 1;
+    } else {
+      c->mbtable_cap = (int64_t)pow2_at_least((uint64_t)(2 * entries));
+      c->mbtable_in_lds = 0;
+    }
+  }
+  ALLOC_TRY(c->d_mbtable, c->mbtable_in_lds ? 1 : (size_t)(c->ring.slots * c->mbtable_cap));
   c->nbr_tcap = (int32_t)pow2_at_least((uint64_t)(2 * (c->nnb + 64)));
   c->nbr_use_lds = (4 * (int64_t)c->nbr_tcap * 8 <= 64 * 1024) ? 1 : 0;
   if (!c->nbr_use_lds) {
@@ -582,7 +626,7 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
     ALLOC_TRY(c->d_nbr_gtable, (size_t)c->nbr_gslots * c->nbr_tcap);
   }
   c->beta_blocks = (int32_t)std::max<int64_t>(
-      1, std::min<int64_t>(256, (c->edge_cap + 4 * rpw * 4 - 1) / (4 * rpw * 4)));
+      1, std::min<int64_t>(128, (c->edge_cap + 4 * rpw * 8 - 1) / (4 * rpw * 8)));
   ALLOC_TRY(c->d_beta_partials, (size_t)c->beta_blocks * kBetaWaves * 2 * c->K);
   c->ppx_blocks = (int32_t)std::max<int64_t>(
       1, std::min<int64_t>(2048, (c->H + 4 * rpw * 8 - 1) / (4 * rpw * 8)));
@@ -784,6 +828,7 @@ int ammsb_set_counters(ammsb_ctx* c, int64_t step_count, int64_t average_count) 
   HIP_TRY(hipSetDevice(c->opt.device));
   c->step_count = step_count;
   c->average_count = average_count;
+  c->ring_valid = false;
   HIP_TRY(launch_set_state_step(c->d_st, step_count, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return AMMSB_OK;
@@ -947,8 +992,9 @@ int ammsb_update_phi(ammsb_ctx* c, const int32_t* nodes, int64_t n_nodes, const 
     pa.ld = c->ld;
     pa.K = c->K;
     pa.N = c->N;
+    pa.use_ring = 0;
+    pa.ring = c->ring;
     pa.nodes = d_nodes;
-    pa.count_dev = nullptr;
     pa.count_host = (int32_t)n_nodes;
     pa.nbrs = d_nbrs;
     pa.nnb = (int32_t)n_nb;
@@ -959,15 +1005,17 @@ int ammsb_update_phi(ammsb_ctx* c, const int32_t* nodes, int64_t n_nodes, const 
     pa.noise = d_noise;
     pa.key = c->key;
     pa.staging = d_stage;
-    e = launch_update_phi(pa, c->prec, (int32_t)n_nodes, c->stream);
+    e = launch_update_phi(pa, c->prec, (int32_t)n_nodes, c->stream, c->phi_waves, c->phi_unroll);
   }
   if (e == hipSuccess) {
     ScatterArgs sc;
     sc.pi = c->d_pi;
     sc.ld = c->ld;
     sc.K = c->K;
+    sc.use_ring = 0;
+    sc.ring = c->ring;
+    sc.st = c->d_st;
     sc.nodes = d_nodes;
-    sc.count_dev = nullptr;
     sc.count_host = (int32_t)n_nodes;
     sc.staging = d_stage;
     e = launch_scatter(sc, c->prec, (int32_t)n_nodes, c->stream);
@@ -1009,9 +1057,11 @@ int ammsb_update_beta(ammsb_ctx* c, const int32_t* edges, const uint8_t* y, int6
       ba.pi = c->d_pi;
       ba.ld = c->ld;
       ba.K = c->K;
+      ba.use_ring = 0;
+      ba.ring = c->ring;
+      ba.st = c->d_st;
       ba.edges = d_edges;
       ba.y = d_y;
-      ba.count_dev = nullptr;
       ba.count_host = (int32_t)E;
       ba.beta = c->d_beta;
       ba.theta = c->d_theta;
@@ -1031,6 +1081,7 @@ int ammsb_update_beta(ammsb_ctx* c, const int32_t* edges, const uint8_t* y, int6
       ta.hp = c->hp;
       ta.eps_override = eps_t;
       ta.scale_override = scale;
+      ta.meta = c->ring.meta;
       ta.noise = d_noise;
       ta.key = c->key;
       ta.bump_step = 0;
@@ -1053,39 +1104,13 @@ int ammsb_debug_sample(ammsb_ctx* c, int64_t step, int32_t* flag, double* scale,
                        int64_t* n_edges, int32_t* nodes, int64_t* n_nodes, int32_t* nbrs) {
   if (!c || step < 1) return AMMSB_ERR_ARG;
   HIP_TRY(hipSetDevice(c->opt.device));
-  const DevGraph g = dev_graph(c);
-  SamplerArgs sa;
-  sa.g = g;
-  sa.st = c->d_st;
-  sa.key = c->key;
-  sa.strategy = c->opt.strategy;
-  sa.m = c->m;
-  sa.max_source = c->opt.sampler_max_source;
-  sa.edge_cap = c->edge_cap;
-  sa.mb_edges = c->d_mb_edges;
-  sa.mb_y = c->d_mb_y;
-  sa.nodes = c->d_nodes;
-  sa.node_cap = c->node_cap;
-  sa.table = c->d_mbtable;
-  sa.table_cap = c->mbtable_cap;
-  sa.stamp = c->d_stamp;
-  sa.step_override = step;
-  HIP_TRY(launch_minibatch(sa, c->stream));
-  NeighborArgs na;
-  na.g = g;
-  na.st = c->d_st;
-  na.key = c->key;
-  na.nodes = c->d_nodes;
-  na.nnb = (int32_t)c->nnb;
-  na.nbrs = c->d_nbrs;
-  na.gtable = c->d_nbr_gtable;
-  na.tcap = c->nbr_tcap;
-  na.use_lds = c->nbr_use_lds;
-  na.step_override = step;
-  na.gtable_slots = c->nbr_gslots;
-  HIP_TRY(launch_neighbors(na, (int32_t)c->node_cap, c->stream));
+  // draw iteration `step` into ring slot 0; the ring is refilled before the
+  // next iteration runs
+  c->ring_valid = false;
+  int rc = fill_ring(c, step, 1);
+  if (rc != AMMSB_OK) return rc;
   DevState h;
-  int rc = read_devstate(c, &h);
+  rc = read_devstate(c, &h);
   if (rc != AMMSB_OK) return rc;
   if (h.error) {
     set_error("ammsb_debug_sample: device sampler error");
@@ -1094,16 +1119,19 @@ int ammsb_debug_sample(ammsb_ctx* c, int64_t step, int32_t* flag, double* scale,
     (void)hipStreamSynchronize(c->stream);
     return AMMSB_ERR_INVARIANT;
   }
-  if (flag) *flag = h.flag;
-  if (scale) *scale = h.scale;
-  if (n_edges) *n_edges = h.mb_count;
-  if (n_nodes) *n_nodes = h.node_count;
-  if (edges && h.mb_count)
-    HIP_TRY(hipMemcpy(edges, c->d_mb_edges, 8 * (size_t)h.mb_count, hipMemcpyDeviceToHost));
-  if (nodes && h.node_count)
-    HIP_TRY(hipMemcpy(nodes, c->d_nodes, 4 * (size_t)h.node_count, hipMemcpyDeviceToHost));
-  if (nbrs && h.node_count)
-    HIP_TRY(hipMemcpy(nbrs, c->d_nbrs, 4 * (size_t)h.node_count * c->nnb, hipMemcpyDeviceToHost));
+  IterMeta mt;
+  HIP_TRY(hipMemcpy(&mt, c->ring.meta, sizeof(IterMeta), hipMemcpyDeviceToHost));
+  if (flag) *flag = mt.flag;
+  if (scale) *scale = mt.scale;
+  if (n_edges) *n_edges = mt.mb_count;
+  if (n_nodes) *n_nodes = mt.node_count;
+  if (edges && mt.mb_count)
+    HIP_TRY(hipMemcpy(edges, c->ring.edges, 8 * (size_t)mt.mb_count, hipMemcpyDeviceToHost));
+  if (nodes && mt.node_count)
+    HIP_TRY(hipMemcpy(nodes, c->ring.nodes, 4 * (size_t)mt.node_count, hipMemcpyDeviceToHost));
+  if (nbrs && mt.node_count)
+    HIP_TRY(hipMemcpy(nbrs, c->ring.nbrs, 4 * (size_t)mt.node_count * c->nnb,
+                      hipMemcpyDeviceToHost));
   return AMMSB_OK;
 }
 
@@ -1115,6 +1143,43 @@ int ammsb_get_totals(ammsb_ctx* c, int64_t* nodes_updated, int64_t* minibatch_ed
   if (rc != AMMSB_OK) return rc;
   if (nodes_updated) *nodes_updated = h.nodes_total;
   if (minibatch_edges) *minibatch_edges = h.edges_total;
+  return AMMSB_OK;
+}
+
+int ammsb_set_tuning(ammsb_ctx* c, const char* name, int64_t value) {
+  if (!c || !name) return AMMSB_ERR_ARG;
+  const std::string n(name);
+  int waves = c->phi_waves, unroll = c->phi_unroll;
+  if (n == "phi_waves")
+    waves = (int)value;
+  else if (n == "phi_unroll")
+    unroll = (int)value;
+  else {
+    set_error("ammsb_set_tuning: unknown parameter " + n);
+    return AMMSB_ERR_ARG;
+  }
+  // a shape that K's row configuration does not offer is rejected here, the
+  // pair is validated when both are set (phi_waves first, then phi_unroll)
+  if (!update_phi_shape_supported(c->K, c->prec, waves, unroll) && n == "phi_unroll") {
+    set_error("ammsb_set_tuning: update_phi shape (waves, unroll) not available for this K");
+    return AMMSB_ERR_ARG;
+  }
+  c->phi_waves = waves;
+  c->phi_unroll = unroll;
+  return AMMSB_OK;
+}
+
+int ammsb_get_tuning(ammsb_ctx* c, const char* name, int64_t* value) {
+  if (!c || !name || !value) return AMMSB_ERR_ARG;
+  const std::string n(name);
+  if (n == "phi_waves")
+    *value = c->phi_waves;
+  else if (n == "phi_unroll")
+    *value = c->phi_unroll;
+  else {
+    set_error("ammsb_get_tuning: unknown parameter " + n);
+    return AMMSB_ERR_ARG;
+  }
   return AMMSB_OK;
 }
 

@@ -17,16 +17,14 @@ namespace ammsb {
 // captured graph).
 struct DevState {
   int64_t step;        // Learner::step_count (1-based)
-  uint32_t epoch;      // bumped by every minibatch draw (node de-dup stamps)
-  int32_t flag;        // 0 = non-link minibatch, 1 = link minibatch
-  int32_t mb_count;    // minibatch edges
-  int32_t node_count;  // unique minibatch nodes
   int32_t error;       // sampler error bits (ERR_*)
   int32_t pad;
-  double scale;        // minibatch unbiasing scale (network.cc:341-342,385-390)
   double ppx;          // last held-out perplexity
   int64_t nodes_total; // sum of node_count over all minibatch draws
   int64_t edges_total; // sum of mb_count over all minibatch draws
+  int32_t done_count;  // k_theta workgroups finished (last one bumps step)
+  int32_t pad2;
+  int64_t ring_base;   // step held by slot 0 of the sample-ahead ring
 };
 
 enum : int32_t {
@@ -55,6 +53,36 @@ struct DevHyper {
   double Nn;  // N / num_node_sample (mcmc_sampler_stochastic.cc:300)
 };
 
+// One sampled iteration (a slot of the sample-ahead ring).
+struct IterMeta {
+  int64_t step;        // iteration whose draws the slot holds
+  int32_t flag;        // 0 = non-link minibatch, 1 = link minibatch
+  int32_t mb_count;    // minibatch edges
+  int32_t node_count;  // unique minibatch nodes
+  uint32_t epoch;      // node de-dup stamp generation of this slot
+  double scale;        // unbiasing scale (network.cc:341-342,385-390)
+};
+
+// Sample-ahead ring: the draws of an iteration depend only on (graph, seed,
+// step), never on the state, so `slots` future iterations are drawn by one
+// launch and consumed one per iteration.  Slot s holds iteration
+// DevState::ring_base + s.
+struct Ring {
+  int32_t slots;
+  int64_t edge_cap;   // edges per slot
+  int64_t node_cap;   // nodes per slot
+  int32_t nnb;        // neighbours per node (n + 1)
+  int32_t N;
+  int32_t* edges;     // [slots][edge_cap][2]
+  uint8_t* y;         // [slots][edge_cap]
+  int32_t* nodes;     // [slots][node_cap]
+  int32_t* nbrs;      // [slots][node_cap][nnb], bit 31 = link
+  IterMeta* meta;     // [slots]
+  uint32_t* stamp;    // [slots][N] node de-dup stamps
+};
+
+__device__ __forceinline__ int ring_slot(const DevState* st) { return (int)(st->step - st->ring_base); }
+
 struct SamplerArgs {
   DevGraph g;
   DevState* st;
@@ -62,28 +90,21 @@ struct SamplerArgs {
   int32_t strategy;
   int64_t m;             // mini_batch_size
   int64_t max_source;
-  int64_t edge_cap;      // capacity of mb_edges
-  int32_t* mb_edges;     // [edge_cap][2]
-  uint8_t* mb_y;         // [edge_cap]
-  int32_t* nodes;        // [node_cap]
-  int64_t node_cap;
-  int2* table;           // non-link de-dup table (global), table_cap entries
-  int64_t table_cap;     // power of two >= 2*(m+1)
-  uint32_t* stamp;       // [N] node de-dup stamps
-  int64_t step_override; // <0: use st->step
+  Ring ring;
+  int64_t t0;            // first step to draw (<0: st->step); block i draws t0+i into slot i
+  int2* table;           // non-link de-dup tables in HBM [slots][table_cap] (if not in LDS)
+  int64_t table_cap;     // power of two, > 1.25 x (m + 1024) entries
+  int32_t table_in_lds;  // 1: the table lives in dynamic LDS instead
 };
 
 struct NeighborArgs {
   DevGraph g;
   DevState* st;
   Key key;
-  const int32_t* nodes;
-  int32_t nnb;            // n + 1
-  int32_t* nbrs;          // [node_cap][nnb], bit 31 = linked
-  int2* gtable;           // global fallback tables [node_cap][tcap] or null
+  Ring ring;
+  int2* gtable;           // global de-dup tables [gtable_slots][tcap] or null
   int32_t tcap;           // per-node table capacity (power of two)
   int32_t use_lds;
-  int64_t step_override;
   int32_t gtable_slots;   // number of global tables (when !use_lds)
 };
 
@@ -92,10 +113,11 @@ struct PhiArgs {
   int64_t ld;
   int32_t K;
   int32_t N;
-  const int32_t* nodes;
-  const int32_t* count_dev;  // node count on the device (or null)
-  int32_t count_host;        // used when count_dev == null
-  const int32_t* nbrs;       // [slot][nnb]
+  int32_t use_ring;          // nodes / nbrs / count of the current ring slot
+  Ring ring;
+  const int32_t* nodes;      // injected path
+  int32_t count_host;        // injected path
+  const int32_t* nbrs;       // injected path [slot][nnb]
   int32_t nnb;
   const double* beta;        // [K]
   const DevState* st;
@@ -110,8 +132,10 @@ struct ScatterArgs {
   void* pi;
   int64_t ld;
   int32_t K;
+  int32_t use_ring;
+  Ring ring;
+  const DevState* st;
   const int32_t* nodes;
-  const int32_t* count_dev;
   int32_t count_host;
   const void* staging;
 };
@@ -120,9 +144,11 @@ struct BetaArgs {
   const void* pi;
   int64_t ld;
   int32_t K;
-  const int32_t* edges;     // [E][2]
-  const uint8_t* y;         // [E]
-  const int32_t* count_dev; // minibatch edge count on device (or null)
+  int32_t use_ring;         // edges / y / count of the current ring slot
+  Ring ring;
+  const DevState* st;
+  const int32_t* edges;     // injected path [E][2]
+  const uint8_t* y;         // injected path [E]
   int32_t count_host;
   const double* beta;
   const double* theta;      // [K][2]
@@ -142,7 +168,8 @@ struct ThetaArgs {
   DevState* st;
   DevHyper hp;
   double eps_override;
-  double scale_override;   // <0: st->scale
+  double scale_override;   // <0: scale of the current ring slot
+  const IterMeta* meta;    // ring metadata (scale, totals) when scale_override < 0
   const double* noise;     // injected [K][2] or null
   Key key;
   int32_t bump_step;       // 1: st->step += 1 at the end (end of an iteration)
@@ -165,12 +192,19 @@ struct PpxArgs {
 };
 
 // launchers (return hipError_t of the launch)
-hipError_t launch_minibatch(const SamplerArgs& a, hipStream_t s);
-hipError_t launch_neighbors(const NeighborArgs& a, int32_t grid_nodes, hipStream_t s);
+// Draw `fills` consecutive iterations into ring slots 0..fills-1.
+hipError_t launch_minibatch(const SamplerArgs& a, int32_t fills, hipStream_t s);
+hipError_t launch_neighbors(const NeighborArgs& a, int32_t fills, hipStream_t s);
 hipError_t launch_update_phi(const PhiArgs& a, int precision, int32_t grid_nodes,
-                             hipStream_t s);
+                             hipStream_t s, int waves, int unroll);
+// Launch shape (wavefronts per node, neighbour rows in flight) available for K.
+bool update_phi_shape_supported(int32_t K, int precision, int waves, int unroll);
+constexpr int kPhiWavesDefault = 4;
+constexpr int kPhiUnrollDefault = 2;
 hipError_t launch_scatter(const ScatterArgs& a, int precision, int32_t grid_nodes,
                           hipStream_t s);
+// Static LDS of k_minibatch (bytes), for sizing its dynamic de-dup table.
+constexpr int kMinibatchStaticLds = 46 * 1024;
 hipError_t launch_beta(const BetaArgs& a, int precision, hipStream_t s);
 hipError_t launch_theta(const ThetaArgs& a, hipStream_t s);
 hipError_t launch_perplexity(const PpxArgs& a, int precision, hipStream_t s);

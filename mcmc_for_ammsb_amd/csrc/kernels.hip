@@ -84,10 +84,53 @@ __device__ __forceinline__ void load_row(const F* row, int K, int lg, F* out) {
   }
 }
 
+// Cross-lane moves through DPP (data-parallel primitives of the VALU, no
+// LDS round trip).  Controls: quad_perm [1,0,3,2] = 0xB1 (lane ^ 1),
+// quad_perm [2,3,0,1] = 0x4E (lane ^ 2), row_half_mirror = 0x141 (lane i of
+// an 8-lane half row <- lane 7-i), row_mirror = 0x140 (lane i of a 16-lane
+// row <- lane 15-i).
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xFFFFFFFFll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float read_lane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double read_lane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xFFFFFFFFll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the LPR lanes of a row group (LPR in {4,8,16,32,64}), result in
+// every lane of the group.  Butterfly inside 16-lane rows with DPP: after
+// the ^1 and ^2 steps a quad is uniform, so the half-row mirror adds the
+// other quad and the row mirror the other half row.  Rows are combined with
+// readlane (uniform scalars).  The summation order is fixed.
 template <int LPR, typename T>
 __device__ __forceinline__ T group_sum(T v) {
-#pragma unroll
-  for (int off = LPR / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  static_assert(LPR == 4 || LPR == 8 || LPR == 16 || LPR == 32 || LPR == 64, "row group");
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  if (LPR >= 8) v += dpp_mov<0x141>(v);
+  if (LPR >= 16) v += dpp_mov<0x140>(v);
+  if (LPR == 32) {
+    const T r0 = read_lane(v, 0), r1 = read_lane(v, 16), r2 = read_lane(v, 32),
+            r3 = read_lane(v, 48);
+    v = ((int)(threadIdx.x & 63) < 32) ? r0 + r1 : r2 + r3;
+  } else if (LPR == 64) {
+    const T r0 = read_lane(v, 0), r1 = read_lane(v, 16), r2 = read_lane(v, 32),
+            r3 = read_lane(v, 48);
+    v = (r0 + r1) + (r2 + r3);
+  }
   return v;
 }
 
@@ -161,8 +204,21 @@ __device__ __forceinline__ int block_scan(int pred, int* s_wave, int* total) {
 }
 
 constexpr int kMbThreads = 1024;
+constexpr int kMbWaves = kMbThreads / 64;
 constexpr int kMaxSources = 1024;
 constexpr int kRowLds = 4096;
+constexpr int kSrcTab = 2 * kMaxSources;  // LDS set of link sources (power of two)
+
+// First source index of node v in the source set, INT_MAX if v is no source.
+__device__ __forceinline__ int src_first(const int2* tab, int32_t v) {
+  uint32_t s = (uint32_t)mix64((uint64_t)(uint32_t)v) & (kSrcTab - 1);
+  while (true) {
+    const int2 e = tab[s];
+    if (e.x == v) return e.y;
+    if (e.x == -1) return INT_MAX;
+    s = (s + 1) & (kSrcTab - 1);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Minibatch sampler: one workgroup.  Sequential definition (the host replica
@@ -176,19 +232,36 @@ constexpr int kRowLds = 4096;
 //       stop if s > 0 and |S| + deg(node) > m; all training edges of node in
 //       CSR order not already in S; repeat the round while S is empty.
 //       scale = N if max_source == 1 else E/|S|           (network.cc:385-390)
-// Attempts are processed 1024 at a time; within a batch, a value is won by
-// its smallest attempt index (atomicMin in the de-dup table), and accepted
-// values are appended in attempt order through a block scan, which yields
-// the sequential result.
+// Non-link attempts are processed 1024 at a time; within a batch a value is
+// won by its smallest attempt index (atomicMin in the de-dup table, kept in
+// LDS when it fits), and accepted values are appended in attempt order
+// through a block scan, which yields the sequential result.
+// Link sources are processed 16 at a time, one wavefront each: an edge of
+// source s is new iff its other end is not a source of index < s (and s is
+// the first occurrence of its node); a single thread then advances |S| over
+// the 16 counts and applies the stop rule, and the kept edges are written at
+// their sequential offsets.
 __global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
-  __shared__ int s_wave[kMbThreads / 64];
-  __shared__ int s_row[kRowLds];
-  __shared__ int s_srcs[kMaxSources];
-  __shared__ int s_nodes;
-  const int tid = threadIdx.x;
+  extern __shared__ int2 s_dyn[];  // non-link de-dup table when table_in_lds
+  __shared__ int s_wave[kMbWaves];
+  __shared__ int s_row[kRowLds];    // non-links: row of the node; links: degrees
+  __shared__ int s_src[kMaxSources];
+  __shared__ int s_kept[kMaxSources];
+  __shared__ int s_cb[kMaxSources];
+  __shared__ int2 s_stab[kSrcTab];
+  __shared__ int s_nodes, s_total, s_used, s_err;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   DevState* st = a.st;
-  const int64_t t = a.step_override >= 0 ? a.step_override : st->step;
+  const int slot = blockIdx.x;
+  const int64_t t0 = a.t0 >= 0 ? a.t0 : st->step;
+  const int64_t t = t0 + slot;
   const uint32_t t32 = (uint32_t)t;
+  const Ring& R = a.ring;
+  int32_t* const mb_edges = R.edges + (int64_t)slot * R.edge_cap * 2;
+  uint8_t* const mb_y = R.y + (int64_t)slot * R.edge_cap;
+  int32_t* const out_nodes = R.nodes + (int64_t)slot * R.node_cap;
+  uint32_t* const stamp = R.stamp + (int64_t)slot * R.N;
+  const int64_t edge_cap = R.edge_cap, node_cap = R.node_cap;
   const uint32_t N = (uint32_t)a.g.N;
   const int64_t m = a.m;
   int err = 0;
@@ -203,7 +276,10 @@ __global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
     flag = -1;
   int64_t c = 0;
   double scale = 0.0;
-  if (tid == 0) s_nodes = 0;
+  if (tid == 0) {
+    s_nodes = 0;
+    s_err = 0;
+  }
   if (flag == 0) {
     const int32_t node = (int32_t)randint_mod(draw_u64(a.key, 0, 0, t32, TAG_SOURCE), N);
     const int64_t rs = a.g.row_ptr[node];
@@ -211,7 +287,8 @@ __global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
     const bool in_lds = deg <= kRowLds;
     if (in_lds)
       for (int64_t i = tid; i < deg; i += kMbThreads) s_row[i] = a.g.cols[rs + i];
-    for (int64_t i = tid; i < a.table_cap; i += kMbThreads) a.table[i] = make_int2(-1, INT_MAX);
+    int2* tab = a.table_in_lds ? s_dyn : a.table + (int64_t)slot * a.table_cap;
+    for (int64_t i = tid; i < a.table_cap; i += kMbThreads) tab[i] = make_int2(-1, INT_MAX);
     __threadfence();
     __syncthreads();
     const uint32_t cmask = (uint32_t)(a.table_cap - 1);
@@ -229,18 +306,19 @@ __global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
       if (valid) valid = hash_find(a.g, lo, hi) < 0;
       int slot = -1;
       if (valid) {
-        slot = table_insert(a.table, cmask, v);
-        atomicMin(&a.table[slot].y, (int)ai);
+        slot = table_insert(tab, cmask, v);
+        atomicMin(&tab[slot].y, (int)ai);
       }
-      __threadfence();
+      if (!a.table_in_lds) __threadfence();
       __syncthreads();
-      const int win = valid && load_agent(&a.table[slot].y) == (int)ai;
+      const int win =
+          valid && (a.table_in_lds ? tab[slot].y : load_agent(&tab[slot].y)) == (int)ai;
       int total;
       const int pos = block_scan(win, s_wave, &total);
       if (win && c + pos < m) {
-        a.mb_edges[2 * (c + pos)] = (int32_t)lo;
-        a.mb_edges[2 * (c + pos) + 1] = (int32_t)hi;
-        a.mb_y[c + pos] = 0;
+        mb_edges[2 * (c + pos)] = (int32_t)lo;
+        mb_edges[2 * (c + pos) + 1] = (int32_t)hi;
+        mb_y[c + pos] = 0;
       }
       c = min(m, c + (int64_t)total);
       if (c >= m) break;
@@ -249,50 +327,104 @@ __global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
     scale = dN * dN / (double)m;
     scale *= dE / (dN * (dN - 1.0) / 2.0 - dE);
   } else if (flag == 1) {
+    int* s_deg = s_row;
+    const int nsrc = (int)min<int64_t>(a.max_source, kMaxSources);
     int64_t r = 0;
     const int64_t max_rounds = 1 << 20;
     for (; r < max_rounds; ++r) {
-      c = 0;
-      int nsrc = 0;
-      for (int64_t s = 0; s < a.max_source; ++s) {
+      for (int i = tid; i < kSrcTab; i += kMbThreads) s_stab[i] = make_int2(-1, INT_MAX);
+      __syncthreads();
+      for (int s = tid; s < nsrc; s += kMbThreads) {
         const int32_t node =
             (int32_t)randint_mod(draw_u64(a.key, (uint32_t)r, (uint32_t)s, t32, TAG_SOURCE), N);
-        const int64_t rs = a.g.row_ptr[node];
-        const int64_t deg = a.g.row_ptr[node + 1] - rs;
-        if (s > 0 && c + deg > m) break;
-        bool dup = false;
-        for (int i = 0; i < nsrc; ++i) dup |= s_srcs[i] == node;
-        __syncthreads();
-        if (tid == 0) s_srcs[nsrc] = node;
-        __syncthreads();
-        ++nsrc;
-        if (dup) continue;
-        if (c + deg > a.edge_cap) {
-          err |= ERR_MB_OVERFLOW;
-          break;
-        }
-        for (int64_t j0 = 0; j0 < deg; j0 += kMbThreads) {
-          const int64_t j = j0 + tid;
-          int keep = 0;
-          uint32_t lo = 0, hi = 0;
-          if (j < deg) {
-            const int32_t v = a.g.cols[rs + j];
-            lo = (uint32_t)min(node, v);
-            hi = (uint32_t)max(node, v);
-            keep = hash_find(a.g, lo, hi) < 0;
-            for (int i = 0; i + 1 < nsrc && keep; ++i) keep = s_srcs[i] != v;
-          }
-          int total;
-          const int pos = block_scan(keep, s_wave, &total);
-          if (keep) {
-            a.mb_edges[2 * (c + pos)] = (int32_t)lo;
-            a.mb_edges[2 * (c + pos) + 1] = (int32_t)hi;
-            a.mb_y[c + pos] = 1;
-          }
-          c += total;
-        }
+        s_src[s] = node;
+        s_deg[s] = (int)(a.g.row_ptr[node + 1] - a.g.row_ptr[node]);
+        const int slot = table_insert(s_stab, kSrcTab - 1, node);
+        atomicMin(&s_stab[slot].y, s);
       }
-      if (c > 0 || err) break;
+      if (tid == 0) {
+        s_total = 0;
+        s_used = -1;
+      }
+      __syncthreads();
+      int used = -1;
+      c = 0;
+      for (int base = 0; base < nsrc && used < 0; base += kMbWaves) {
+        const int s = base + w;
+        if (s < nsrc) {
+          const int32_t node = s_src[s];
+          int kept = 0;
+          if (src_first(s_stab, node) == s) {
+            const int64_t rs = a.g.row_ptr[node];
+            const int deg = s_deg[s];
+            for (int j0 = 0; j0 < deg; j0 += 64) {
+              const int j = j0 + lane;
+              bool keep = false;
+              if (j < deg) {
+                const int32_t v = a.g.cols[rs + j];
+                keep = src_first(s_stab, v) >= s &&
+                       hash_find(a.g, (uint32_t)min(node, v), (uint32_t)max(node, v)) < 0;
+              }
+              kept += __popcll(__ballot(keep));
+            }
+          }
+          if (lane == 0) s_kept[s] = kept;
+        }
+        __syncthreads();
+        if (tid == 0) {
+          int64_t cc = s_total;
+          int u = -1;
+          const int hi = min(base + kMbWaves, nsrc);
+          for (int q = base; q < hi; ++q) {
+            if (q > 0 && cc + s_deg[q] > m) {
+              u = q;
+              break;
+            }
+            if (cc + s_kept[q] > edge_cap) {
+              s_err |= ERR_MB_OVERFLOW;
+              u = q;
+              break;
+            }
+            s_cb[q] = (int)cc;
+            cc += s_kept[q];
+          }
+          s_total = (int)cc;
+          s_used = u;
+        }
+        __syncthreads();
+        c = s_total;
+        used = s_used;
+      }
+      if (used < 0) used = nsrc;
+      if (c > 0) {
+        for (int s = w; s < used; s += kMbWaves) {
+          const int32_t node = s_src[s];
+          if (src_first(s_stab, node) != s) continue;
+          const int64_t rs = a.g.row_ptr[node];
+          const int deg = s_deg[s];
+          int off = s_cb[s];
+          for (int j0 = 0; j0 < deg; j0 += 64) {
+            const int j = j0 + lane;
+            bool keep = false;
+            int32_t v = 0;
+            if (j < deg) {
+              v = a.g.cols[rs + j];
+              keep = src_first(s_stab, v) >= s &&
+                     hash_find(a.g, (uint32_t)min(node, v), (uint32_t)max(node, v)) < 0;
+            }
+            const uint64_t mk = __ballot(keep);
+            if (keep) {
+              const int pos = off + __popcll(mk & ((1ull << lane) - 1ull));
+              mb_edges[2 * pos] = min(node, v);
+              mb_edges[2 * pos + 1] = max(node, v);
+              mb_y[pos] = 1;
+            }
+            off += __popcll(mk);
+          }
+        }
+        break;
+      }
+      __syncthreads();
     }
     if (r >= max_rounds) err |= ERR_LINK_ROUNDS;
     scale = a.max_source == 1 ? (double)a.g.N : (double)a.g.E / (double)(c > 0 ? c : 1);
@@ -304,30 +436,40 @@ __global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
   // nodes_in_batch: unique endpoints (mcmc_sampler_stochastic.cc:315-327).
   // Order is not part of the contract: every per-node draw is keyed by the
   // node id, not by its position.
-  const uint32_t epoch = st->epoch + 1u;
-  for (int64_t e = tid; e < c; e += kMbThreads) {
+  const uint32_t epoch = R.meta[slot].epoch + 1u;
+  for (int64_t e0 = 0; e0 < c; e0 += kMbThreads) {
+    const int64_t e = e0 + tid;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      const int32_t v = a.mb_edges[2 * e + q];
-      if (atomicExch(&a.stamp[v], epoch) != epoch) {
-        const int pos = atomicAdd(&s_nodes, 1);
-        if (pos < a.node_cap) a.nodes[pos] = v;
+      int32_t v = 0;
+      bool isnew = false;
+      if (e < c) {
+        v = mb_edges[2 * e + q];
+        isnew = atomicExch(&stamp[v], epoch) != epoch;
       }
+      // one LDS atomic per wavefront, positions by ballot prefix
+      const uint64_t mk = __ballot(isnew);
+      int base = 0;
+      if (lane == 0 && mk) base = atomicAdd(&s_nodes, __popcll(mk));
+      base = __shfl(base, 0, 64);
+      const int pos = base + __popcll(mk & ((1ull << lane) - 1ull));
+      if (isnew && pos < node_cap) out_nodes[pos] = v;
     }
   }
   __syncthreads();
   if (tid == 0) {
-    if (s_nodes > a.node_cap) err |= ERR_MB_OVERFLOW;
-    st->epoch = epoch;
-    st->flag = flag;
-    st->mb_count = (int32_t)c;
-    st->node_count = min(s_nodes, (int)a.node_cap);
-    st->scale = scale;
-    st->error |= err;
-    if (a.step_override < 0) {
-      st->nodes_total += min(s_nodes, (int)a.node_cap);
-      st->edges_total += c;
-    }
+    err |= s_err;
+    if (s_nodes > node_cap) err |= ERR_MB_OVERFLOW;
+    IterMeta mt;
+    mt.step = t;
+    mt.flag = flag;
+    mt.mb_count = (int32_t)c;
+    mt.node_count = (int32_t)min<int64_t>(s_nodes, node_cap);
+    mt.epoch = epoch;
+    mt.scale = scale;
+    R.meta[slot] = mt;
+    if (err) atomicOr(&st->error, err);
+    if (slot == 0) st->ring_base = t0;
   }
 }
 
@@ -340,16 +482,17 @@ __global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
 // update_phi, mcmc_sampler_stochastic.cc:279-283).
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void k_neighbors(NeighborArgs a, int32_t slot_offset,
-                                                          int32_t slot_end) {
+                                                          int32_t slot_end, int32_t rslot0) {
   extern __shared__ int2 s_tab[];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  const int slot = slot_offset + blockIdx.x * WAVES + w;
-  const int count = a.st->node_count;
+  const int rslot = rslot0 + blockIdx.y;  // ring slot (iteration)
+  const int slot = slot_offset + blockIdx.x * WAVES + w;  // node slot
+  const IterMeta& mt = a.ring.meta[rslot];
+  const int count = mt.node_count;
   if (slot >= count || slot >= slot_end) return;
-  const int64_t t = a.step_override >= 0 ? a.step_override : a.st->step;
-  const uint32_t t32 = (uint32_t)t;
+  const uint32_t t32 = (uint32_t)mt.step;
   const uint32_t N = (uint32_t)a.g.N;
-  const int32_t node = a.nodes[slot];
+  const int32_t node = a.ring.nodes[(int64_t)rslot * a.ring.node_cap + slot];
   int2* tab = a.use_lds ? s_tab + (size_t)w * a.tcap
                         : a.gtable + (size_t)(slot - slot_offset) * a.tcap;
   for (int i = lane; i < a.tcap; i += 64) tab[i] = make_int2(-1, INT_MAX);
@@ -358,10 +501,11 @@ __global__ __launch_bounds__(WAVES * 64) void k_neighbors(NeighborArgs a, int32_
   const uint32_t cmask = (uint32_t)(a.tcap - 1);
   const int64_t rs = a.g.row_ptr[node];
   const int64_t deg = a.g.row_ptr[node + 1] - rs;
-  int32_t* out = a.nbrs + (int64_t)slot * a.nnb;
+  const int nnb = a.ring.nnb;
+  int32_t* out = a.ring.nbrs + ((int64_t)rslot * a.ring.node_cap + slot) * nnb;
   int c = 0;
-  const int max_batches = 4096 + 64 * ((a.nnb + 63) / 64);
-  for (int batch = 0; c < a.nnb; ++batch) {
+  const int max_batches = 4096 + 64 * ((nnb + 63) / 64);
+  for (int batch = 0; c < nnb; ++batch) {
     if (batch >= max_batches) {
       if (lane == 0) atomicOr(&a.st->error, ERR_NEIGHBOR_ATTEMPTS);
       break;
@@ -380,7 +524,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_neighbors(NeighborArgs a, int32_
     const bool win = valid && (a.use_lds ? tab[s].y : load_agent(&tab[s].y)) == (int)ai;
     const uint64_t mk = __ballot(win);
     const int pos = c + __popcll(mk & ((1ull << lane) - 1ull));
-    if (win && pos < a.nnb) {
+    if (win && pos < nnb) {
       const bool lk = row_contains(a.g.cols + rs, deg, v);
       out[pos] = v | (lk ? (int32_t)0x80000000 : 0);
     }
@@ -410,13 +554,25 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
   constexpr int STRIDE = WAVES * RPW;
   __shared__ F s_acc[(WAVES > 1 ? WAVES - 1 : 1) * EPL * LPR];
   const int slot = blockIdx.x;
-  const int count = a.count_dev ? *a.count_dev : a.count_host;
+  int count;
+  const int32_t* nodes;
+  const int32_t* nbrs;
+  if (a.use_ring) {
+    const int rs = ring_slot(a.st);
+    count = a.ring.meta[rs].node_count;
+    nodes = a.ring.nodes + (int64_t)rs * a.ring.node_cap;
+    nbrs = a.ring.nbrs + (int64_t)rs * a.ring.node_cap * a.nnb;
+  } else {
+    count = a.count_host;
+    nodes = a.nodes;
+    nbrs = a.nbrs;
+  }
   if (slot >= count) return;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int g = lane / LPR, lg = lane % LPR;
   const F* pi = (const F*)a.pi;
   const int K = a.K;
-  const int32_t node = a.nodes[slot];
+  const int32_t node = nodes[slot];
   const F* prow = pi + (int64_t)node * a.ld;
   const F eps = (F)a.hp.epsilon;
 
@@ -433,7 +589,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
       acc[c * V + v] = (F)0;
     }
   const F e1 = eps, e0 = (F)1 - eps;
-  const int32_t* nb_base = a.nbrs + (int64_t)slot * a.nnb;
+  const int32_t* nb_base = nbrs + (int64_t)slot * a.nnb;
 
   for (int b0 = w * RPW + g; b0 < a.nnb; b0 += U * STRIDE) {
     F row[U][EPL];
@@ -538,13 +694,22 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
 // Publish staged rows: pi[nodes[slot]] = staging[slot] (K+1 elements).
 template <typename F>
 __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
-  const int count = a.count_dev ? *a.count_dev : a.count_host;
+  int count;
+  const int32_t* nodes;
+  if (a.use_ring) {
+    const int rs = ring_slot(a.st);
+    count = a.ring.meta[rs].node_count;
+    nodes = a.ring.nodes + (int64_t)rs * a.ring.node_cap;
+  } else {
+    count = a.count_host;
+    nodes = a.nodes;
+  }
   const int slot = blockIdx.x;
   if (slot >= count) return;
   constexpr int V = VT<F>::V;
   const int nv = (a.K + 1 + V - 1) / V;
   const F* src = (const F*)a.staging + (int64_t)slot * a.ld;
-  F* dst = (F*)a.pi + (int64_t)a.nodes[slot] * a.ld;
+  F* dst = (F*)a.pi + (int64_t)nodes[slot] * a.ld;
   typedef typename VT<F>::type T;
   for (int i = threadIdx.x; i < nv; i += blockDim.x)
     reinterpret_cast<T*>(dst)[i] = reinterpret_cast<const T*>(src)[i];
@@ -567,7 +732,19 @@ __global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
   const int K = a.K;
   const F* pi = (const F*)a.pi;
   const F eps = (F)a.epsilon;
-  const int count = a.count_dev ? *a.count_dev : a.count_host;
+  int count;
+  const int32_t* edges;
+  const uint8_t* ys;
+  if (a.use_ring) {
+    const int rs = ring_slot(a.st);
+    count = a.ring.meta[rs].mb_count;
+    edges = a.ring.edges + (int64_t)rs * a.ring.edge_cap * 2;
+    ys = a.ring.y + (int64_t)rs * a.ring.edge_cap;
+  } else {
+    count = a.count_host;
+    edges = a.edges;
+    ys = a.y;
+  }
   F bt[EPL];
   double c0y[EPL], c0n[EPL], c1y[EPL], c1n[EPL], g0[EPL], g1[EPL];
 #pragma unroll
@@ -592,8 +769,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
     }
   const int gstride = a.blocks * WAVES * RPW;
   for (int e = (blockIdx.x * WAVES + w) * RPW + g; e < count; e += gstride) {
-    const int32_t ea = a.edges[2 * e], eb = a.edges[2 * e + 1];
-    const int y = a.y[e] ? 1 : 0;
+    const int32_t ea = edges[2 * e], eb = edges[2 * e + 1];
+    const int y = ys[e] ? 1 : 0;
     F ra[EPL], rb[EPL];
     load_row<F, LPR, CH>(pi + (int64_t)ea * a.ld, K, lg, ra);
     load_row<F, LPR, CH>(pi + (int64_t)eb * a.ld, K, lg, rb);
@@ -638,19 +815,39 @@ __global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
     }
 }
 
-// theta step (mcmc_sampler_stochastic.cc:245-266), one workgroup.
-__global__ __launch_bounds__(256) void k_theta(ThetaArgs a) {
+// theta step (mcmc_sampler_stochastic.cc:245-266).  Workgroup b owns
+// k in [64b, 64b+64); its 16 wavefronts sum strided partial rows, then the
+// 16 sums meet in LDS in a fixed order (deterministic).  The last workgroup
+// to finish advances step_count, so every workgroup reads the same step.
+constexpr int kThetaThreads = 1024;
+__global__ __launch_bounds__(kThetaThreads) void k_theta(ThetaArgs a) {
+  __shared__ double s_g[kThetaThreads / 64][2][64];
   DevState* st = a.st;
   const int64_t t = st->step;
-  const double eps_t = a.eps_override >= 0.0 ? a.eps_override : eps_of_step(a.hp, t);
-  const double scale = a.scale_override >= 0.0 ? a.scale_override : st->scale;
+  const int tid = threadIdx.x, kl = tid & 63, rg = tid >> 6;
   const int K = a.K;
-  for (int k = threadIdx.x; k < K; k += blockDim.x) {
-    double G0 = 0.0, G1 = 0.0;
-    for (int b = 0; b < a.blocks; ++b) {
+  const int k = blockIdx.x * 64 + kl;
+  double G0 = 0.0, G1 = 0.0;
+  if (k < K) {
+#pragma unroll 8
+    for (int b = rg; b < a.blocks; b += kThetaThreads / 64) {
       G0 += a.partials[(int64_t)b * 2 * K + k];
       G1 += a.partials[(int64_t)b * 2 * K + K + k];
     }
+  }
+  s_g[rg][0][kl] = G0;
+  s_g[rg][1][kl] = G1;
+  __syncthreads();
+  if (rg == 0 && k < K) {
+    G0 = 0.0;
+    G1 = 0.0;
+    for (int q = 0; q < kThetaThreads / 64; ++q) {
+      G0 += s_g[q][0][kl];
+      G1 += s_g[q][1][kl];
+    }
+    const double eps_t = a.eps_override >= 0.0 ? a.eps_override : eps_of_step(a.hp, t);
+    const double scale =
+        a.scale_override >= 0.0 ? a.scale_override : a.meta[ring_slot(st)].scale;
     double xi0, xi1;
     if (a.noise) {
       xi0 = a.noise[2 * k];
@@ -675,7 +872,16 @@ __global__ __launch_bounds__(256) void k_theta(ThetaArgs a) {
   }
   if (a.bump_step) {
     __syncthreads();
-    if (threadIdx.x == 0) st->step = t + 1;
+    if (tid == 0) {
+      __threadfence();
+      if (atomicAdd(&st->done_count, 1) == (int)gridDim.x - 1) {
+        const IterMeta& mt = a.meta[ring_slot(st)];
+        st->nodes_total += mt.node_count;
+        st->edges_total += mt.mb_count;
+        st->done_count = 0;
+        st->step = t + 1;
+      }
+    }
   }
 }
 
@@ -865,41 +1071,77 @@ bool row_config(int32_t K, int precision, int* lpr, int* ch) {
   return false;
 }
 
-hipError_t launch_minibatch(const SamplerArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_minibatch, dim3(1), dim3(kMbThreads), 0, s, a);
+hipError_t launch_minibatch(const SamplerArgs& a, int32_t fills, hipStream_t s) {
+  if (fills < 1 || fills > a.ring.slots) return hipErrorInvalidValue;
+  const size_t lds = a.table_in_lds ? (size_t)a.table_cap * sizeof(int2) : 0;
+  hipLaunchKernelGGL(k_minibatch, dim3(fills), dim3(kMbThreads), lds, s, a);
   return hipGetLastError();
 }
 
-hipError_t launch_neighbors(const NeighborArgs& a, int32_t grid_nodes, hipStream_t s) {
+hipError_t launch_neighbors(const NeighborArgs& a, int32_t fills, hipStream_t s) {
+  const int32_t grid_nodes = (int32_t)a.ring.node_cap;
   if (a.use_lds) {
     const size_t lds = (size_t)kNbrWaves * a.tcap * sizeof(int2);
     const int blocks = (grid_nodes + kNbrWaves - 1) / kNbrWaves;
-    hipLaunchKernelGGL(k_neighbors<kNbrWaves>, dim3(blocks), dim3(kNbrWaves * 64), lds, s, a, 0,
-                       grid_nodes);
+    hipLaunchKernelGGL(k_neighbors<kNbrWaves>, dim3(blocks, fills), dim3(kNbrWaves * 64), lds, s,
+                       a, 0, grid_nodes, 0);
   } else {
     // global de-dup tables exist for gtable_slots nodes at a time
-    for (int32_t off = 0; off < grid_nodes; off += a.gtable_slots) {
-      const int n = std::min<int32_t>(a.gtable_slots, grid_nodes - off);
-      hipLaunchKernelGGL(k_neighbors<1>, dim3(n), dim3(64), 0, s, a, off, off + n);
-    }
+    for (int32_t rs = 0; rs < fills; ++rs)
+      for (int32_t off = 0; off < grid_nodes; off += a.gtable_slots) {
+        const int n = std::min<int32_t>(a.gtable_slots, grid_nodes - off);
+        hipLaunchKernelGGL(k_neighbors<1>, dim3(n, 1), dim3(64), 0, s, a, off, off + n, rs);
+      }
   }
   return hipGetLastError();
 }
 
+namespace {
+// Launch shapes of k_update_phi: WAVES wavefronts per node, U neighbour rows
+// in flight per row group.  Full-wave rows with CH in {1,2,4,8} (the common
+// K of FP64 K<=1024 and FP32 K<=2048) get every shape; other row configs
+// run the default (4, 2).
+bool phi_shape_ok(int lpr, int ch, int waves, int unroll) {
+  if (waves == kPhiWaves && unroll == kPhiUnroll) return true;
+  if (!(lpr == 64 && (ch == 1 || ch == 2 || ch == 4 || ch == 8))) return false;
+  return (waves == 2 && (unroll == 2 || unroll == 4)) ||
+         (waves == 4 && (unroll == 1 || unroll == 4)) ||
+         (waves == 8 && (unroll == 1 || unroll == 2 || unroll == 4));
+}
+
+template <typename F, int L, int C>
+hipError_t phi_launch(const PhiArgs& a, int grid, hipStream_t s, int waves, int unroll) {
+#define V(W, U)                                                                        \
+  if (waves == W && unroll == U) {                                                     \
+    hipLaunchKernelGGL((k_update_phi<F, L, C, W, U>), dim3(grid), dim3(W * 64), 0, s, a); \
+    return hipGetLastError();                                                          \
+  }
+  V(kPhiWaves, kPhiUnroll)
+  if constexpr (L == 64 && (C == 1 || C == 2 || C == 4 || C == 8)) {
+    V(2, 2) V(2, 4) V(4, 1) V(4, 4) V(8, 1) V(8, 2) V(8, 4)
+  }
+#undef V
+  return hipErrorInvalidValue;
+}
+}  // namespace
+
+bool update_phi_shape_supported(int32_t K, int precision, int waves, int unroll) {
+  int lpr, ch;
+  if (!row_config(K, precision, &lpr, &ch)) return false;
+  return phi_shape_ok(lpr, ch, waves, unroll);
+}
+
 hipError_t launch_update_phi(const PhiArgs& a, int precision, int32_t grid_nodes,
-                             hipStream_t s) {
+                             hipStream_t s, int waves, int unroll) {
   int lpr, ch;
   if (!row_config(a.K, precision, &lpr, &ch)) return hipErrorInvalidValue;
+  if (!phi_shape_ok(lpr, ch, waves, unroll)) return hipErrorInvalidValue;
   if (grid_nodes <= 0) return hipSuccess;
-#define X(L, C)                                                                           \
-  if (lpr == L && ch == C) {                                                              \
-    if (precision == 0)                                                                   \
-      hipLaunchKernelGGL((k_update_phi<double, L, C, kPhiWaves, kPhiUnroll>),             \
-                         dim3(grid_nodes), dim3(kPhiWaves * 64), 0, s, a);                \
-    else                                                                                  \
-      hipLaunchKernelGGL((k_update_phi<float, L, C, kPhiWaves, kPhiUnroll>),              \
-                         dim3(grid_nodes), dim3(kPhiWaves * 64), 0, s, a);                \
-    return hipGetLastError();                                                             \
+#define X(L, C)                                                        \
+  if (lpr == L && ch == C) {                                           \
+    if (precision == 0)                                                \
+      return phi_launch<double, L, C>(a, grid_nodes, s, waves, unroll); \
+    return phi_launch<float, L, C>(a, grid_nodes, s, waves, unroll);    \
   }
   AMMSB_ROW_CONFIGS(X)
 #undef X
@@ -935,7 +1177,7 @@ hipError_t launch_beta(const BetaArgs& a, int precision, hipStream_t s) {
 }
 
 hipError_t launch_theta(const ThetaArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_theta, dim3(1), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_theta, dim3((a.K + 63) / 64), dim3(kThetaThreads), 0, s, a);
   return hipGetLastError();
 }
 

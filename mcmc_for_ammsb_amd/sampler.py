@@ -183,6 +183,22 @@ class MCMCSamplerStochastic:
                                          _lib.ptr(ticks, ctypes.c_int64)))
         return {p: (float(s), int(t)) for p, s, t in zip(PHASES, secs, ticks)}
 
+    def set_tuning(self, **kw):
+        """Launch shape of update_phi, e.g. set_tuning(phi_waves=8, phi_unroll=2)."""
+        for name in ("phi_waves", "phi_unroll"):
+            if name in kw:
+                _lib.check(_lib.lib.ammsb_set_tuning(self._h, name.encode(), int(kw.pop(name))))
+        if kw:
+            raise ValueError(f"unknown tuning parameters {sorted(kw)}")
+
+    def get_tuning(self):
+        out = {}
+        for name in ("phi_waves", "phi_unroll"):
+            v = ctypes.c_int64()
+            _lib.check(_lib.lib.ammsb_get_tuning(self._h, name.encode(), ctypes.byref(v)))
+            out[name] = v.value
+        return out
+
     def totals(self):
         """(nodes updated, minibatch edges) summed over all iterations."""
         n, e = ctypes.c_int64(), ctypes.c_int64()
