@@ -280,7 +280,7 @@ int ammsb_get_totals(ammsb_ctx *ctx, int64_t *nodes_updated,
  *   "phi_unroll" neighbour rows in flight per row group (1, 2, 4)
  * Set phi_waves first; the (waves, unroll) pair is validated when
  * phi_unroll is set.  Results do not depend on the shape beyond summation
- * order.  Defaults (4, 2). */
+ * order.  Defaults (4, 1). */
 int ammsb_set_tuning(ammsb_ctx *ctx, const char *name, int64_t value);
 int ammsb_get_tuning(ammsb_ctx *ctx, const char *name, int64_t *value);
 

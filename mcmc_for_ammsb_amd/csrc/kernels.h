@@ -200,7 +200,7 @@ hipError_t launch_update_phi(const PhiArgs& a, int precision, int32_t grid_nodes
 // Launch shape (wavefronts per node, neighbour rows in flight) available for K.
 bool update_phi_shape_supported(int32_t K, int precision, int waves, int unroll);
 constexpr int kPhiWavesDefault = 4;
-constexpr int kPhiUnrollDefault = 2;
+constexpr int kPhiUnrollDefault = 1;
 hipError_t launch_scatter(const ScatterArgs& a, int precision, int32_t grid_nodes,
                           hipStream_t s);
 // Static LDS of k_minibatch (bytes), for sizing its dynamic de-dup table.

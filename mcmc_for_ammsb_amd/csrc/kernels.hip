@@ -65,23 +65,52 @@ __device__ __forceinline__ void load_vec<float>(const float* p, float* out) {
   out[3] = v.w;
 }
 
-// Row of K elements into registers: vector vi = c*LPR + lg, masked to k < K.
+// Row of K elements into registers: vector vi = c*LPR + lg.  The loads are
+// unconditional so that every vector of a row (and of several rows) is in
+// flight before the first wait: a vector past the end of the row re-reads
+// vector 0 of the same row (in bounds) and is zeroed by mask_row().
 template <typename F, int LPR, int CH>
-__device__ __forceinline__ void load_row(const F* row, int K, int lg, F* out) {
+__device__ __forceinline__ void issue_row(const F* row, int K, int lg, F* out) {
   constexpr int V = VT<F>::V;
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     const int k0 = (c * LPR + lg) * V;
-    if (k0 < K) {
-      load_vec<F>(row + k0, out + c * V);
-#pragma unroll
-      for (int v = 0; v < V; ++v)
-        if (k0 + v >= K) out[c * V + v] = (F)0;
-    } else {
-#pragma unroll
-      for (int v = 0; v < V; ++v) out[c * V + v] = (F)0;
-    }
+    load_vec<F>(row + (k0 < K ? k0 : 0), out + c * V);
   }
+}
+// Zero the elements k >= K (selects, no branches around memory operations).
+template <typename F, int LPR, int CH>
+__device__ __forceinline__ void mask_row(int K, int lg, F* out) {
+  constexpr int V = VT<F>::V;
+  if (CH * LPR * V <= K) return;  // every lane's vectors lie inside the row
+#pragma unroll
+  for (int c = 0; c < CH; ++c)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int k = (c * LPR + lg) * V + v;
+      out[c * V + v] = k < K ? out[c * V + v] : (F)0;
+    }
+}
+template <typename F, int LPR, int CH>
+__device__ __forceinline__ void load_row(const F* row, int K, int lg, F* out) {
+  issue_row<F, LPR, CH>(row, K, lg, out);
+  mask_row<F, LPR, CH>(K, lg, out);
+}
+
+// 1/x: hardware reciprocal + two Newton-Raphson steps (FP64), or the
+// hardware reciprocal + one step (FP32); error within an ulp or two, against
+// the reference's exact divide.
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ float recip(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  const float e = fmaf(-x, r, 1.0f);
+  return fmaf(r, e, r);
 }
 
 // Cross-lane moves through DPP (data-parallel primitives of the VALU, no
@@ -203,6 +232,7 @@ __device__ __forceinline__ int block_scan(int pred, int* s_wave, int* total) {
   return off + within;
 }
 
+constexpr int kPhiNbLds = 2048;  // neighbour ids staged in LDS by k_update_phi
 constexpr int kMbThreads = 1024;
 constexpr int kMbWaves = kMbThreads / 64;
 constexpr int kMaxSources = 1024;
@@ -553,6 +583,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
   constexpr int RPW = 64 / LPR;
   constexpr int STRIDE = WAVES * RPW;
   __shared__ F s_acc[(WAVES > 1 ? WAVES - 1 : 1) * EPL * LPR];
+  __shared__ int32_t s_nb[kPhiNbLds];
   const int slot = blockIdx.x;
   int count;
   const int32_t* nodes;
@@ -589,36 +620,52 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
       acc[c * V + v] = (F)0;
     }
   const F e1 = eps, e0 = (F)1 - eps;
-  const int32_t* nb_base = nbrs + (int64_t)slot * a.nnb;
-
-  for (int b0 = w * RPW + g; b0 < a.nnb; b0 += U * STRIDE) {
-    F row[U][EPL];
-    int lab[U];
-    bool ok[U];
+  // The node's neighbour ids (bit 31 = link) are staged in LDS, chunk by
+  // chunk, and the U ids of an iteration are read before any of its rows is
+  // requested, so that the U rows' loads are all in flight together.
+  const int32_t* nb_src = nbrs + (int64_t)slot * a.nnb;
+  for (int cb = 0; cb < a.nnb; cb += kPhiNbLds) {
+    const int cn = min(kPhiNbLds, a.nnb - cb);
+    __syncthreads();
+    for (int i = tid; i < cn; i += WAVES * 64) s_nb[i] = nb_src[cb + i];
+    __syncthreads();
+    for (int b0 = w * RPW + g; b0 < cn; b0 += U * STRIDE) {
+      F row[U][EPL];
+      int32_t nbv[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int b = b0 + u * STRIDE;
-      const int32_t nb = b < a.nnb ? nb_base[b] : node;
-      const int32_t v = nb & 0x7fffffff;
-      ok[u] = (b < a.nnb) && (v != node);
-      lab[u] = (int)((uint32_t)nb >> 31);
-      load_row<F, LPR, CH>(pi + (int64_t)v * a.ld, K, lg, row[u]);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const F e = lab[u] ? e1 : e0;
-      F p = (F)0;
-#pragma unroll
-      for (int j = 0; j < EPL; ++j) {
-        const F f = lab[u] ? f1[j] : f0[j];
-        row[u][j] = row[u][j] * f + e;  // x_bk
-        p += pii[j] * row[u][j];
+      for (int u = 0; u < U; ++u) {
+        const int b = b0 + u * STRIDE;
+        nbv[u] = b < cn ? s_nb[b] : node;  // past the end: own row, not counted
       }
-      p = group_sum<LPR>(p);
-      const F wz = (F)1 / p;
-      if (ok[u]) {
 #pragma unroll
-        for (int j = 0; j < EPL; ++j) acc[j] += row[u][j] * wz - (F)1;
+      for (int u = 0; u < U; ++u) {
+        if (LPR == 64) nbv[u] = __builtin_amdgcn_readfirstlane(nbv[u]);  // one row per wavefront
+        issue_row<F, LPR, CH>(pi + (int64_t)(nbv[u] & 0x7fffffff) * a.ld, K, lg, row[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        mask_row<F, LPR, CH>(K, lg, row[u]);
+        const bool ok = (b0 + u * STRIDE < cn) && ((nbv[u] & 0x7fffffff) != node);
+        F p = (F)0;
+        if (nbv[u] < 0) {  // link: f = beta - eps, e = eps
+#pragma unroll
+          for (int j = 0; j < EPL; ++j) {
+            row[u][j] = fma(row[u][j], f1[j], e1);  // x_bk
+            p = fma(pii[j], row[u][j], p);
+          }
+        } else {  // non-link: f = eps - beta, e = 1 - eps
+#pragma unroll
+          for (int j = 0; j < EPL; ++j) {
+            row[u][j] = fma(row[u][j], f0[j], e0);
+            p = fma(pii[j], row[u][j], p);
+          }
+        }
+        p = group_sum<LPR>(p);
+        const F wz = recip(p);
+        if (ok) {
+#pragma unroll
+          for (int j = 0; j < EPL; ++j) acc[j] += fma(row[u][j], wz, (F)-1);
+        }
       }
     }
   }
@@ -772,8 +819,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
     const int32_t ea = edges[2 * e], eb = edges[2 * e + 1];
     const int y = ys[e] ? 1 : 0;
     F ra[EPL], rb[EPL];
-    load_row<F, LPR, CH>(pi + (int64_t)ea * a.ld, K, lg, ra);
-    load_row<F, LPR, CH>(pi + (int64_t)eb * a.ld, K, lg, rb);
+    issue_row<F, LPR, CH>(pi + (int64_t)ea * a.ld, K, lg, ra);
+    issue_row<F, LPR, CH>(pi + (int64_t)eb * a.ld, K, lg, rb);
+    mask_row<F, LPR, CH>(K, lg, ra);
+    mask_row<F, LPR, CH>(K, lg, rb);
     F pisum = (F)0, psum = (F)0;
 #pragma unroll
     for (int j = 0; j < EPL; ++j) {
@@ -915,8 +964,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_ppx(PpxArgs a) {
     const int32_t ea = a.edges[2 * e], eb = a.edges[2 * e + 1];
     const bool y = a.labels[e] != 0;
     F ra[EPL], rb[EPL];
-    load_row<F, LPR, CH>(pi + (int64_t)ea * a.ld, K, lg, ra);
-    load_row<F, LPR, CH>(pi + (int64_t)eb * a.ld, K, lg, rb);
+    issue_row<F, LPR, CH>(pi + (int64_t)ea * a.ld, K, lg, ra);
+    issue_row<F, LPR, CH>(pi + (int64_t)eb * a.ld, K, lg, rb);
+    mask_row<F, LPR, CH>(K, lg, ra);
+    mask_row<F, LPR, CH>(K, lg, rb);
     F s = (F)0, sum = (F)0;
 #pragma unroll
     for (int j = 0; j < EPL; ++j) {
@@ -1026,7 +1077,7 @@ __global__ void k_set_step(DevState* st, int64_t step) {
 }
 
 constexpr int kPhiWaves = 4;
-constexpr int kPhiUnroll = 2;
+constexpr int kPhiUnroll = 1;
 
 constexpr int kPpxWaves = 4;
 constexpr int kNbrWaves = 4;
@@ -1100,12 +1151,12 @@ namespace {
 // Launch shapes of k_update_phi: WAVES wavefronts per node, U neighbour rows
 // in flight per row group.  Full-wave rows with CH in {1,2,4,8} (the common
 // K of FP64 K<=1024 and FP32 K<=2048) get every shape; other row configs
-// run the default (4, 2).
+// run the default (4, 1).
 bool phi_shape_ok(int lpr, int ch, int waves, int unroll) {
   if (waves == kPhiWaves && unroll == kPhiUnroll) return true;
   if (!(lpr == 64 && (ch == 1 || ch == 2 || ch == 4 || ch == 8))) return false;
   return (waves == 2 && (unroll == 2 || unroll == 4)) ||
-         (waves == 4 && (unroll == 1 || unroll == 4)) ||
+         (waves == 4 && (unroll == 2 || unroll == 4)) ||
          (waves == 8 && (unroll == 1 || unroll == 2 || unroll == 4));
 }
 
@@ -1118,7 +1169,7 @@ hipError_t phi_launch(const PhiArgs& a, int grid, hipStream_t s, int waves, int 
   }
   V(kPhiWaves, kPhiUnroll)
   if constexpr (L == 64 && (C == 1 || C == 2 || C == 4 || C == 8)) {
-    V(2, 2) V(2, 4) V(4, 1) V(4, 4) V(8, 1) V(8, 2) V(8, 4)
+    V(2, 2) V(2, 4) V(4, 2) V(4, 4) V(8, 1) V(8, 2) V(8, 4)
   }
 #undef V
   return hipErrorInvalidValue;
