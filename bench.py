@@ -169,6 +169,8 @@ def main():
         except (OSError, ValueError):
             traffic = None
 
+    probe_s, probe_bytes = smp.probe_read(5)  # contiguous read of the resident state
+    log(f"contiguous read of {probe_bytes / 1e9:.2f} GB: {probe_bytes / probe_s / 1e9:.0f} GB/s")
     log("phase            total_s   ticks   per_tick_us")
     for k, (sec, n) in ph.items():
         log(f"{k:<16} {sec:9.4f} {n:7d} {1e6 * sec / max(n, 1):12.2f}")
@@ -201,6 +203,7 @@ def main():
         "update_phi_kernel_node_k_per_s": nodes * K / phi_s if phi_s > 0 else None,
         "phase_us_per_step": {k: 1e6 * sec / args.steps for k, (sec, n) in ph.items()},
         "perplexity_last": ckpt[-1][1] if ckpt else None,
+        "stream_read_GBs": probe_bytes / probe_s / 1e9,
         "cpu_baseline": None,
     }
     if rank == 0 and not args.no_cpu_baseline:

@@ -284,6 +284,11 @@ int ammsb_get_totals(ammsb_ctx *ctx, int64_t *nodes_updated,
 int ammsb_set_tuning(ammsb_ctx *ctx, const char *name, int64_t value);
 int ammsb_get_tuning(ammsb_ctx *ctx, const char *name, int64_t *value);
 
+/* Bandwidth probe (no counterpart in the reference): `reps` contiguous reads
+ * of the resident pi array; *seconds = average time of one read, *bytes =
+ * bytes read.  Also a read of known size for calibrating HBM counters. */
+int ammsb_probe_read(ammsb_ctx *ctx, int32_t reps, double *seconds, int64_t *bytes);
+
 /* Device pointers of the resident state for interop (torch DLPack-free
  * views): pi ([N][ld] of the precision type), theta ([K][2] double),
  * beta ([K] double). */

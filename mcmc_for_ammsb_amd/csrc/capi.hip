@@ -316,7 +316,7 @@ int enqueue_iteration(ammsb_ctx* c) {
     ta.beta = c->d_beta;
     ta.K = c->K;
     ta.partials = c->d_beta_partials;
-    ta.blocks = c->beta_blocks * kBetaWaves;
+    ta.blocks = c->beta_blocks;
     ta.st = c->d_st;
     ta.hp = c->hp;
     ta.eps_override = -1.0;
@@ -626,8 +626,8 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
     ALLOC_TRY(c->d_nbr_gtable, (size_t)c->nbr_gslots * c->nbr_tcap);
   }
   c->beta_blocks = (int32_t)std::max<int64_t>(
-      1, std::min<int64_t>(128, (c->edge_cap + 4 * rpw * 8 - 1) / (4 * rpw * 8)));
-  ALLOC_TRY(c->d_beta_partials, (size_t)c->beta_blocks * kBetaWaves * 2 * c->K);
+      1, std::min<int64_t>(256, (c->edge_cap + 4 * rpw * 4 - 1) / (4 * rpw * 4)));
+  ALLOC_TRY(c->d_beta_partials, (size_t)c->beta_blocks * 2 * c->K);
   c->ppx_blocks = (int32_t)std::max<int64_t>(
       1, std::min<int64_t>(2048, (c->H + 4 * rpw * 8 - 1) / (4 * rpw * 8)));
   ALLOC_TRY(c->d_ppx_partials, (size_t)c->ppx_blocks * 4);
@@ -1076,7 +1076,7 @@ int ammsb_update_beta(ammsb_ctx* c, const int32_t* edges, const uint8_t* y, int6
       ta.beta = c->d_beta;
       ta.K = c->K;
       ta.partials = c->d_beta_partials;
-      ta.blocks = c->beta_blocks * kBetaWaves;
+      ta.blocks = c->beta_blocks;
       ta.st = c->d_st;
       ta.hp = c->hp;
       ta.eps_override = eps_t;
@@ -1180,6 +1180,29 @@ int ammsb_get_tuning(ammsb_ctx* c, const char* name, int64_t* value) {
     set_error("ammsb_get_tuning: unknown parameter " + n);
     return AMMSB_ERR_ARG;
   }
+  return AMMSB_OK;
+}
+
+int ammsb_probe_read(ammsb_ctx* c, int32_t reps, double* seconds, int64_t* bytes) {
+  if (!c || reps < 1 || !seconds || !bytes) return AMMSB_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->opt.device));
+  const int64_t n = ((int64_t)c->N * c->ld * (int64_t)c->fsize) / 16 * 16;
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) {
+    set_error("ammsb_probe_read: event creation failed");
+    return AMMSB_ERR_HIP;
+  }
+  HIP_TRY(launch_read_probe(c->d_pi, n, c->d_ppx_partials, c->stream));  // warm
+  HIP_TRY(hipEventRecord(a, c->stream));
+  for (int32_t r = 0; r < reps; ++r) HIP_TRY(launch_read_probe(c->d_pi, n, c->d_ppx_partials, c->stream));
+  HIP_TRY(hipEventRecord(b, c->stream));
+  HIP_TRY(hipEventSynchronize(b));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, a, b));
+  c->event_pool.push_back(a);
+  c->event_pool.push_back(b);
+  *seconds = ms * 1e-3 / reps;
+  *bytes = n;
   return AMMSB_OK;
 }
 

@@ -153,7 +153,7 @@ struct BetaArgs {
   const double* beta;
   const double* theta;      // [K][2]
   double epsilon;
-  double* partials;         // [blocks * kBetaWaves][2K]
+  double* partials;         // [blocks][2K]
   int32_t blocks;
 };
 
@@ -223,4 +223,9 @@ namespace ammsb {
 // (injected-draw path of update_phi).
 hipError_t launch_label_nbrs(DevGraph g, const int32_t* nodes, int32_t n_nodes, int32_t* nbrs,
                              int32_t nnb, hipStream_t s);
+}  // namespace ammsb
+
+namespace ammsb {
+// Contiguous read of `bytes` (a multiple of 16) starting at p.
+hipError_t launch_read_probe(const void* p, int64_t bytes, double* sink, hipStream_t s);
 }  // namespace ammsb

@@ -607,7 +607,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
   const F* prow = pi + (int64_t)node * a.ld;
   const F eps = (F)a.hp.epsilon;
 
-  F pii[EPL], f1[EPL], f0[EPL], acc[EPL];
+  F pii[EPL], f1[EPL], acc[EPL];
   load_row<F, LPR, CH>(prow, K, lg, pii);
 #pragma unroll
   for (int c = 0; c < CH; ++c)
@@ -615,8 +615,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
     for (int v = 0; v < V; ++v) {
       const int k = (c * LPR + lg) * V + v;
       const F bk = k < K ? (F)a.beta[k] : (F)0;
-      f1[c * V + v] = bk - eps;
-      f0[c * V + v] = eps - bk;
+      f1[c * V + v] = bk - eps;  // link factor; the non-link factor is -f1
       acc[c * V + v] = (F)0;
     }
   const F e1 = eps, e0 = (F)1 - eps;
@@ -656,7 +655,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
         } else {  // non-link: f = eps - beta, e = 1 - eps
 #pragma unroll
           for (int j = 0; j < EPL; ++j) {
-            row[u][j] = fma(row[u][j], f0[j], e0);
+            row[u][j] = fma(-row[u][j], f1[j], e0);
             p = fma(pii[j], row[u][j], p);
           }
         }
@@ -768,12 +767,13 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
 //   prob_0 = (eps or 1-eps) (1 - sum_k f_k), Z = sum_k probs_k + prob_0
 //   G_k0 += probs_k/Z ((1-y)/theta_k0 - 1/S_k)
 //   G_k1 += probs_k/Z (y/theta_k1 - 1/S_k),   S_k = theta_k0 + theta_k1
-// accumulated in double; one partial [2][K] per wavefront.
+// accumulated in double; one partial [2][K] per workgroup.
 template <typename F, int LPR, int CH, int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
   constexpr int V = VT<F>::V;
   constexpr int EPL = V * CH;
   constexpr int RPW = 64 / LPR;
+  __shared__ double s_g[(WAVES > 1 ? WAVES - 1 : 1) * EPL * LPR];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int g = lane / LPR, lg = lane % LPR;
   const int K = a.K;
@@ -792,54 +792,74 @@ __global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
     edges = a.edges;
     ys = a.y;
   }
+  // g1[j] / g0[j]: sum of probs_k/Z over the edges with y = 1 / y = 0; the
+  // theta factors ((1-y)/theta_k0 - 1/S_k, y/theta_k1 - 1/S_k) depend on k
+  // and y only, and are applied once when the partial row is written.
   F bt[EPL];
-  double c0y[EPL], c0n[EPL], c1y[EPL], c1n[EPL], g0[EPL], g1[EPL];
+  double g0[EPL], g1[EPL];
 #pragma unroll
   for (int c = 0; c < CH; ++c)
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       const int k = (c * LPR + lg) * V + v;
       const int j = c * V + v;
-      double th0 = 1.0, th1 = 1.0;
-      if (k < K) {
-        th0 = a.theta[2 * k];
-        th1 = a.theta[2 * k + 1];
-      }
-      const double inv_s = 1.0 / (th0 + th1);
       bt[j] = k < K ? (F)a.beta[k] : (F)0;
-      c0y[j] = 0.0 / th0 - inv_s;
-      c0n[j] = 1.0 / th0 - inv_s;
-      c1y[j] = 1.0 / th1 - inv_s;
-      c1n[j] = 0.0 / th1 - inv_s;
       g0[j] = 0.0;
       g1[j] = 0.0;
     }
+  // Two edges per row group in flight: both edges' ids are read, then all
+  // four rows are requested, then the edges are reduced.
+  constexpr int UE = CH <= 4 ? 2 : 1;  // big rows: one edge at a time (registers)
   const int gstride = a.blocks * WAVES * RPW;
-  for (int e = (blockIdx.x * WAVES + w) * RPW + g; e < count; e += gstride) {
-    const int32_t ea = edges[2 * e], eb = edges[2 * e + 1];
-    const int y = ys[e] ? 1 : 0;
-    F ra[EPL], rb[EPL];
-    issue_row<F, LPR, CH>(pi + (int64_t)ea * a.ld, K, lg, ra);
-    issue_row<F, LPR, CH>(pi + (int64_t)eb * a.ld, K, lg, rb);
-    mask_row<F, LPR, CH>(K, lg, ra);
-    mask_row<F, LPR, CH>(K, lg, rb);
-    F pisum = (F)0, psum = (F)0;
+  for (int e0 = (blockIdx.x * WAVES + w) * RPW + g; e0 < count; e0 += UE * gstride) {
+    int32_t ea[UE], eb[UE];
+    int yv[UE];
+    bool val[UE];
 #pragma unroll
-    for (int j = 0; j < EPL; ++j) {
-      const F f = ra[j] * rb[j];
-      pisum += f;
-      ra[j] = y ? bt[j] * f : ((F)1 - bt[j]) * f;  // probs_k
-      psum += ra[j];
+    for (int u = 0; u < UE; ++u) {
+      const int e = e0 + u * gstride;
+      val[u] = e < count;
+      const int ee = val[u] ? e : e0;
+      ea[u] = edges[2 * ee];
+      eb[u] = edges[2 * ee + 1];
+      yv[u] = ys[ee] ? 1 : 0;
     }
-    pisum = group_sum<LPR>(pisum);
-    psum = group_sum<LPR>(psum);
-    const F prob0 = (y ? eps : (F)1 - eps) * ((F)1 - pisum);
-    const F Z = psum + prob0;
+    F ra[UE][EPL], rb[UE][EPL];
 #pragma unroll
-    for (int j = 0; j < EPL; ++j) {
-      const double f = (double)(ra[j] / Z);
-      g0[j] += f * (y ? c0y[j] : c0n[j]);
-      g1[j] += f * (y ? c1y[j] : c1n[j]);
+    for (int u = 0; u < UE; ++u) {
+      if (LPR == 64) {
+        ea[u] = __builtin_amdgcn_readfirstlane(ea[u]);
+        eb[u] = __builtin_amdgcn_readfirstlane(eb[u]);
+        yv[u] = __builtin_amdgcn_readfirstlane(yv[u]);
+      }
+      issue_row<F, LPR, CH>(pi + (int64_t)ea[u] * a.ld, K, lg, ra[u]);
+      issue_row<F, LPR, CH>(pi + (int64_t)eb[u] * a.ld, K, lg, rb[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < UE; ++u) {
+      if (!val[u]) continue;  // uniform within the row group
+      mask_row<F, LPR, CH>(K, lg, ra[u]);
+      mask_row<F, LPR, CH>(K, lg, rb[u]);
+      const int y = yv[u];
+      F pisum = (F)0, psum = (F)0;
+#pragma unroll
+      for (int j = 0; j < EPL; ++j) {
+        const F f = ra[u][j] * rb[u][j];
+        pisum += f;
+        ra[u][j] = y ? bt[j] * f : ((F)1 - bt[j]) * f;  // probs_k
+        psum += ra[u][j];
+      }
+      pisum = group_sum<LPR>(pisum);
+      psum = group_sum<LPR>(psum);
+      const F prob0 = (y ? eps : (F)1 - eps) * ((F)1 - pisum);
+      const F rz = recip(psum + prob0);
+      if (y) {
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) g1[j] += (double)(ra[u][j] * rz);
+      } else {
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) g0[j] += (double)(ra[u][j] * rz);
+      }
     }
   }
   if (RPW > 1) {
@@ -849,17 +869,48 @@ __global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
       g1[j] = cross_group_sum<LPR>(g1[j]);
     }
   }
-  if (g != 0) return;
-  // one partial row per wavefront: [blocks * WAVES][2K]
-  double* out = a.partials + ((int64_t)blockIdx.x * WAVES + w) * 2 * K;
+  // wavefront partials meet in LDS (g0 then g1 through one buffer): one
+  // partial row [2K] per workgroup
+  if (WAVES > 1) {
+    if (w > 0 && g == 0) {
+#pragma unroll
+      for (int j = 0; j < EPL; ++j) s_g[((w - 1) * EPL + j) * LPR + lg] = g0[j];
+    }
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+      for (int q = 1; q < WAVES; ++q)
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) g0[j] += s_g[((q - 1) * EPL + j) * LPR + lg];
+    }
+    __syncthreads();
+    if (w > 0 && g == 0) {
+#pragma unroll
+      for (int j = 0; j < EPL; ++j) s_g[((w - 1) * EPL + j) * LPR + lg] = g1[j];
+    }
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+      for (int q = 1; q < WAVES; ++q)
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) g1[j] += s_g[((q - 1) * EPL + j) * LPR + lg];
+    }
+  }
+  if (w != 0 || g != 0) return;
+  double* out = a.partials + (int64_t)blockIdx.x * 2 * K;
 #pragma unroll
   for (int c = 0; c < CH; ++c)
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       const int k = (c * LPR + lg) * V + v;
       if (k < K) {
-        out[k] = g0[c * V + v];
-        out[K + k] = g1[c * V + v];
+        const int j = c * V + v;
+        const double th0 = a.theta[2 * k], th1 = a.theta[2 * k + 1];
+        const double inv_s = 1.0 / (th0 + th1);
+        // G_k0 = sum_{y=1} f (0/th0 - 1/S) + sum_{y=0} f (1/th0 - 1/S)
+        // G_k1 = sum_{y=1} f (1/th1 - 1/S) + sum_{y=0} f (0/th1 - 1/S)
+        out[k] = g1[j] * (0.0 / th0 - inv_s) + g0[j] * (1.0 / th0 - inv_s);
+        out[K + k] = g1[j] * (1.0 / th1 - inv_s) + g0[j] * (0.0 / th1 - inv_s);
       }
     }
 }
@@ -959,36 +1010,64 @@ __global__ __launch_bounds__(WAVES * 64) void k_ppx(PpxArgs a) {
     }
   double ll1 = 0.0, ll0 = 0.0, n1 = 0.0, n0 = 0.0;
   const double cnt = a.avg_count;
+  // Two held-out edges per row group in flight: ids, labels and running
+  // averages first, then all four rows, then the reductions.
+  constexpr int UE = CH <= 4 ? 2 : 1;  // big rows: one edge at a time (registers)
   const int64_t gstride = (int64_t)a.blocks * WAVES * RPW;
-  for (int64_t e = ((int64_t)blockIdx.x * WAVES + w) * RPW + g; e < a.H; e += gstride) {
-    const int32_t ea = a.edges[2 * e], eb = a.edges[2 * e + 1];
-    const bool y = a.labels[e] != 0;
-    F ra[EPL], rb[EPL];
-    issue_row<F, LPR, CH>(pi + (int64_t)ea * a.ld, K, lg, ra);
-    issue_row<F, LPR, CH>(pi + (int64_t)eb * a.ld, K, lg, rb);
-    mask_row<F, LPR, CH>(K, lg, ra);
-    mask_row<F, LPR, CH>(K, lg, rb);
-    F s = (F)0, sum = (F)0;
+  for (int64_t e0 = ((int64_t)blockIdx.x * WAVES + w) * RPW + g; e0 < a.H; e0 += UE * gstride) {
+    int32_t ea[UE], eb[UE];
+    int yv[UE];
+    bool val[UE];
+    double rold[UE];
 #pragma unroll
-    for (int j = 0; j < EPL; ++j) {
-      const F f = ra[j] * rb[j];
-      s += y ? f * bt[j] : f * ((F)1 - bt[j]);
-      sum += f;
+    for (int u = 0; u < UE; ++u) {
+      const int64_t e = e0 + u * gstride;
+      val[u] = e < a.H;
+      const int64_t ee = val[u] ? e : e0;
+      ea[u] = a.edges[2 * ee];
+      eb[u] = a.edges[2 * ee + 1];
+      yv[u] = a.labels[ee] != 0;
+      rold[u] = lg == 0 ? a.R[ee] : 0.0;
     }
-    s = group_sum<LPR>(s);
-    sum = group_sum<LPR>(sum);
-    if (!y) s += ((F)1 - sum) * ((F)1 - eps);
-    if (s < (F)1e-30) s = (F)1e-30;
-    if (lg == 0) {
-      const double r = (a.R[e] * (cnt - 1.0) + (double)s) / cnt;
-      a.R[e] = r;
-      const double lr = log(r);
-      if (y) {
-        ll1 += lr;
-        n1 += 1.0;
-      } else {
-        ll0 += lr;
-        n0 += 1.0;
+    F ra[UE][EPL], rb[UE][EPL];
+#pragma unroll
+    for (int u = 0; u < UE; ++u) {
+      if (LPR == 64) {
+        ea[u] = __builtin_amdgcn_readfirstlane(ea[u]);
+        eb[u] = __builtin_amdgcn_readfirstlane(eb[u]);
+        yv[u] = __builtin_amdgcn_readfirstlane(yv[u]);
+      }
+      issue_row<F, LPR, CH>(pi + (int64_t)ea[u] * a.ld, K, lg, ra[u]);
+      issue_row<F, LPR, CH>(pi + (int64_t)eb[u] * a.ld, K, lg, rb[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < UE; ++u) {
+      if (!val[u]) continue;  // uniform within the row group
+      mask_row<F, LPR, CH>(K, lg, ra[u]);
+      mask_row<F, LPR, CH>(K, lg, rb[u]);
+      const bool y = yv[u] != 0;
+      F s = (F)0, sum = (F)0;
+#pragma unroll
+      for (int j = 0; j < EPL; ++j) {
+        const F f = ra[u][j] * rb[u][j];
+        s += y ? f * bt[j] : f * ((F)1 - bt[j]);
+        sum += f;
+      }
+      s = group_sum<LPR>(s);
+      sum = group_sum<LPR>(sum);
+      if (!y) s += ((F)1 - sum) * ((F)1 - eps);
+      if (s < (F)1e-30) s = (F)1e-30;
+      if (lg == 0) {
+        const double r = (rold[u] * (cnt - 1.0) + (double)s) / cnt;
+        a.R[e0 + u * gstride] = r;
+        const double lr = log(r);
+        if (y) {
+          ll1 += lr;
+          n1 += 1.0;
+        } else {
+          ll0 += lr;
+          n0 += 1.0;
+        }
       }
     }
   }
@@ -1305,6 +1384,30 @@ hipError_t launch_label_nbrs(DevGraph g, const int32_t* nodes, int32_t n_nodes, 
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_label_nbrs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, nodes,
                      n_nodes, nbrs, nnb);
+  return hipGetLastError();
+}
+}  // namespace ammsb
+
+namespace ammsb {
+namespace {
+// Contiguous read of `n` 16-byte vectors (bandwidth probe and counter
+// calibration: a read of known size).  The sum is stored only if it equals
+// an impossible value, so the loads cannot be removed.
+__global__ __launch_bounds__(256) void k_read_probe(const double2* __restrict__ p, int64_t n,
+                                                    double* sink) {
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = p[i];
+    s += v.x + v.y;
+  }
+  if (s == -1.2345678e300) sink[0] = s;
+}
+}  // namespace
+
+hipError_t launch_read_probe(const void* p, int64_t bytes, double* sink, hipStream_t s) {
+  const int64_t n = bytes / 16;
+  hipLaunchKernelGGL(k_read_probe, dim3(256 * 64), dim3(256), 0, s, (const double2*)p, n, sink);
   return hipGetLastError();
 }
 }  // namespace ammsb

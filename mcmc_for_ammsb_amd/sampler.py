@@ -199,6 +199,12 @@ class MCMCSamplerStochastic:
             out[name] = v.value
         return out
 
+    def probe_read(self, reps=5):
+        """(seconds, bytes) of one contiguous read of the resident pi array."""
+        sec, nb = ctypes.c_double(), ctypes.c_int64()
+        _lib.check(_lib.lib.ammsb_probe_read(self._h, int(reps), ctypes.byref(sec), ctypes.byref(nb)))
+        return sec.value, nb.value
+
     def totals(self):
         """(nodes updated, minibatch edges) summed over all iterations."""
         n, e = ctypes.c_int64(), ctypes.c_int64()
