@@ -103,6 +103,96 @@ def cpu_baseline(g, s, opts, pi, theta, start_step, budget_s, threads):
     }
 
 
+def trajectory(smp, steps, interval):
+    """Checkpoints (step, wall s including perplexity, wall s excluding it, perplexity),
+    perplexity before the first step and then every `interval` steps
+    (mcmc_sampler_stochastic.cc:112-130)."""
+    out = []
+    t_all, t_ppx = 0.0, 0.0
+    done = 0
+    while True:
+        a = time.perf_counter()
+        p = smp.perplexity()
+        t_ppx += time.perf_counter() - a
+        t_all += time.perf_counter() - a
+        out.append((done + 1, t_all, t_all - t_ppx, p))
+        if done >= steps:
+            break
+        a = time.perf_counter()
+        smp.step(interval, sync=True)
+        t_all += time.perf_counter() - a
+        done += interval
+    return out
+
+
+def run_ttt(args):
+    """Time-to-target of the FP32 precision mode against the FP64 path on the
+    same workload, initial state and Philox seed; error per quantity."""
+    import mcmc_for_ammsb_amd as M
+    w = dict(WORKLOADS[args.workload])
+    w["precision"] = "fp64"
+    g, s, opts = build_workload(w, args.seed)
+    res = {}
+    states = {}
+    sample_nodes = np.sort(np.random.default_rng(0).choice(g.N, size=min(g.N, 65536), replace=False))
+    for prec in ("fp64", "fp32"):
+        o = M.Options(**{**opts.__dict__, "precision": prec, "remains": []})
+        smp = M.MCMCSamplerStochastic(o, g, s)
+        smp.init(o.random_seed)
+        smp.step(2)  # device warm-up on the same draws as the run (state reset below)
+        smp.init(o.random_seed)
+        smp.set_counters(1, 1)
+        traj = trajectory(smp, args.ttt_steps, args.ttt_interval)
+        rows = smp.get_rows(sample_nodes).astype(np.float64)  # 65,536 random rows
+        _, _, beta = smp.get_state() if g.N * (opts.K + 1) <= 50_000_000 else (None, None, None)
+        if beta is None:
+            beta = np.empty(opts.K)
+            import ctypes
+            from mcmc_for_ammsb_amd import _lib
+            _lib.check(_lib.lib.ammsb_get_state(smp._h, None, None, _lib.ptr(beta, ctypes.c_double)))
+        states[prec] = (rows, beta)
+        res[prec] = traj
+        log(f"{prec}: " + " ".join(f"{t}:{p:.5f}@{a:.2f}s" for t, a, _, p in traj))
+        smp.close()
+    target = res["fp64"][-1][3]
+    def reach(traj):
+        for t, a, b, p in traj:
+            if abs(p - target) <= 0.01 * target or p <= target:
+                return t, a, b
+        return None, None, None
+    t64 = reach(res["fp64"])
+    t32 = reach(res["fp32"])
+    pi64, b64 = states["fp64"]
+    pi32, b32 = states["fp32"]
+    K = pi64.shape[1] - 1
+    dpi = np.abs(pi32[:, :K] - pi64[:, :K])
+    rel_ppx = [abs(a[3] - b[3]) / b[3] for a, b in zip(res["fp32"], res["fp64"])]
+    out = {
+        "metric": "time-to-target held-out perplexity, FP32 precision mode vs FP64",
+        "workload": f"{args.workload}: {w['desc']} (FP32 and FP64 from the same state and seed)",
+        "steps": args.ttt_steps, "interval": args.ttt_interval,
+        "target_perplexity": target,
+        "fp64_time_to_target_s": t64[1], "fp64_time_to_target_s_excl_perplexity": t64[2],
+        "fp64_steps_to_target": t64[0],
+        "fp32_time_to_target_s": t32[1], "fp32_time_to_target_s_excl_perplexity": t32[2],
+        "fp32_steps_to_target": t32[0],
+        "speedup": (t64[1] / t32[1]) if t32[1] else None,
+        "speedup_excl_perplexity": (t64[2] / t32[2]) if t32[2] else None,
+        "errors_vs_fp64": {
+            "rows_compared": int(len(sample_nodes)),
+            "pi_max_abs": float(dpi.max()), "pi_mean_abs": float(dpi.mean()),
+            "pi_max_rel": float((dpi / np.maximum(pi64[:, :K], 1e-30)).max()),
+            "phi_sum_max_rel": float(np.abs(pi32[:, K] / pi64[:, K] - 1).max()),
+            "beta_max_rel": float(np.abs(b32 / b64 - 1).max()),
+            "beta_mean_rel": float(np.abs(b32 / b64 - 1).mean()),
+            "perplexity_max_rel": float(max(rel_ppx)), "perplexity_mean_rel": float(np.mean(rel_ppx)),
+            "perplexity_final_rel": rel_ppx[-1]},
+        "fp64_checkpoints": res["fp64"], "fp32_checkpoints": res["fp32"],
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,12 +206,17 @@ def main():
     ap.add_argument("--phi-waves", type=int, default=0, help="update_phi wavefronts per node")
     ap.add_argument("--phi-unroll", type=int, default=0, help="update_phi rows in flight")
     ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "update_phi_traffic.json"))
+    ap.add_argument("--mode", default="throughput", choices=["throughput", "ttt"])
+    ap.add_argument("--ttt-steps", type=int, default=3000)
+    ap.add_argument("--ttt-interval", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         log("bench.py measures one GPU: this build has no multi-GPU path "
             "(every box has a single MI355X); run with --gpus 1")
         return 2
     rank = int(os.environ.get("RANK", "0"))
+    if args.mode == "ttt":
+        return run_ttt(args)
 
     import mcmc_for_ammsb_amd as M
     w = WORKLOADS[args.workload]

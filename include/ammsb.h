@@ -289,6 +289,32 @@ int ammsb_get_tuning(ammsb_ctx *ctx, const char *name, int64_t *value);
  * bytes read.  Also a read of known size for calibrating HBM counters. */
 int ammsb_probe_read(ammsb_ctx *ctx, int32_t reps, double *seconds, int64_t *bytes);
 
+/* ---- differentiation --------------------------------------------------- */
+/* Minibatch log-likelihood of M source nodes, all pointers on the device:
+ *   L = sum_s sum_{p in [pair_ptr[s], pair_ptr[s+1])} log p(y_p | pi_s, pi_{b_p}, beta)
+ *   p(y) = sum_k pi_sk (pi_bk f_k + e), f_k = beta_k - eps / eps - beta_k,
+ *          e = eps / 1 - eps for y = 1 / 0
+ * -- the per-pair likelihood of update_phi (mcmc_sampler_stochastic.cc:285-292)
+ * and update_beta (:220-233).  pi_s = phi_s / sum(phi_s) from the caller's
+ * rows phi [M][phi_ld] (precision of the context), neighbour rows b_p from the
+ * resident state (constants, as in update_phi), beta_k = theta_k1 /
+ * (theta_k0 + theta_k1) from theta [K][2] (NULL: the resident theta).  A pair
+ * whose neighbour equals nodes[s] is skipped (:275-277).  P = number of pairs.
+ * fwd: per_source [M] (may be NULL) and total (may be NULL), FP64.
+ * bwd: the adjoint, recomputing each pair's likelihood:
+ *   grad_phi [M][phi_ld] = gout * dL/dphi   (update_phi's grads, :293-295)
+ *   grad_theta [K][2]    = gout * dL/dtheta (update_beta's G, :234-239; may be NULL)
+ * gout points to one double.  stream: a HIP stream to run on (NULL: the
+ * context's stream); the call returns after the work is done. */
+int ammsb_loglik_fwd(ammsb_ctx *ctx, int64_t M, const int32_t *nodes, const void *phi,
+                     int64_t phi_ld, const double *theta, const int64_t *pair_ptr,
+                     const int32_t *pair_nbr, const uint8_t *pair_y, int64_t P,
+                     double *per_source, double *total, void *stream);
+int ammsb_loglik_bwd(ammsb_ctx *ctx, int64_t M, const int32_t *nodes, const void *phi,
+                     int64_t phi_ld, const double *theta, const int64_t *pair_ptr,
+                     const int32_t *pair_nbr, const uint8_t *pair_y, int64_t P,
+                     const double *gout, void *grad_phi, double *grad_theta, void *stream);
+
 /* Device pointers of the resident state for interop (torch DLPack-free
  * views): pi ([N][ld] of the precision type), theta ([K][2] double),
  * beta ([K] double). */

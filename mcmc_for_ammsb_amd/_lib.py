@@ -92,11 +92,41 @@ class AmmsbError(RuntimeError):
         self.code = code
 
 
+def _share_torch_hip_runtime():
+    """PyTorch-ROCm wheels bundle their own libamdhip64 (soname
+    libamdhip64.so.7, the same as /opt/rocm's).  Two HIP runtimes in one
+    process each claim the GPU and the second one finds none, and device
+    pointers and streams are only meaningful inside one runtime.  Mapping
+    torch's copy before libammsb.so makes the loader satisfy our
+    libamdhip64.so.7 dependency with it (match by soname), so the sampler and
+    torch tensors share one runtime whichever is imported first.  torch itself
+    is not imported.  AMMSB_HIP_RUNTIME=system keeps /opt/rocm's runtime."""
+    if os.environ.get("AMMSB_HIP_RUNTIME", "torch") == "system":
+        return None
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return None
+    if spec is None or spec.origin is None:
+        return None
+    path = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if not os.path.exists(path):
+        return None
+    ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    return path
+
+
+HIP_RUNTIME = None
+
+
 def _load():
+    global HIP_RUNTIME
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python mcmc_for_ammsb_amd/build.py` "
             "(the sampler has no CPU fallback)")
+    HIP_RUNTIME = _share_torch_hip_runtime()
     lib = ctypes.CDLL(LIB_PATH)
     sig = {
         "ammsb_last_error": (ctypes.c_char_p, []),
@@ -128,6 +158,10 @@ def _load():
         "ammsb_debug_sample": (c_i32, [c_vp, c_i64, P_i32, P_dbl, P_i32, P_i64, P_i32, P_i64,
                                        P_i32]),
         "ammsb_get_totals": (c_i32, [c_vp, P_i64, P_i64]),
+        "ammsb_loglik_fwd": (c_i32, [c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_i64,
+                                     c_vp, c_vp, c_vp]),
+        "ammsb_loglik_bwd": (c_i32, [c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_i64,
+                                     c_vp, c_vp, c_vp, c_vp]),
         "ammsb_probe_read": (c_i32, [c_vp, c_i32, P_dbl, P_i64]),
         "ammsb_set_tuning": (c_i32, [c_vp, ctypes.c_char_p, c_i64]),
         "ammsb_get_tuning": (c_i32, [c_vp, ctypes.c_char_p, P_i64]),

@@ -94,6 +94,8 @@ struct ammsb_ctx {
   int64_t mbtable_cap = 0;
   int32_t mbtable_in_lds = 0;
   int32_t phi_waves = kPhiWavesDefault, phi_unroll = kPhiUnrollDefault;
+  double* d_ll_scratch = nullptr;   // differentiation: per-source values / dbeta partials
+  size_t ll_scratch_bytes = 0;
   int2* d_nbr_gtable = nullptr;
   int32_t nbr_tcap = 0, nbr_use_lds = 1, nbr_gslots = 0;
   double* d_beta_partials = nullptr;
@@ -413,7 +415,7 @@ void free_ctx(ammsb_ctx* c) {
                   c->d_ho_labels, c->d_R,     c->d_pi,       c->d_theta,     c->d_beta,
                   c->d_st,      c->ring.edges, c->ring.y,    c->ring.nodes,  c->ring.nbrs,
                   c->d_staging, c->d_mbtable, c->ring.stamp, c->d_nbr_gtable, c->d_beta_partials,
-                  c->ring.meta,
+                  c->ring.meta, c->d_ll_scratch,
                   c->d_ppx_partials};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1204,6 +1206,96 @@ int ammsb_probe_read(ammsb_ctx* c, int32_t reps, double* seconds, int64_t* bytes
   *seconds = ms * 1e-3 / reps;
   *bytes = n;
   return AMMSB_OK;
+}
+
+namespace {
+int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi, int64_t phi_ld,
+                  const double* theta, const int64_t* pair_ptr, const int32_t* pair_nbr,
+                  const uint8_t* pair_y, int64_t P, bool bwd, double* per_source, double* total,
+                  const double* gout, void* grad_phi, double* grad_theta, void* stream) {
+  if (!c || M < 0 || P < 0 || (M > 0 && (!nodes || !phi || !pair_ptr)) || (P > 0 && (!pair_nbr || !pair_y)) ||
+      phi_ld < c->K) {
+    set_error("ammsb_loglik: bad arguments");
+    return AMMSB_ERR_ARG;
+  }
+  if (!bwd && (!total && !per_source)) return AMMSB_ERR_ARG;
+  if (bwd && (!gout || (!grad_phi && !grad_theta))) return AMMSB_ERR_ARG;
+  if (M > INT32_MAX) return AMMSB_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->opt.device));
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  if (stream) HIP_TRY(hipStreamSynchronize(c->stream));  // resident state settled
+  const int32_t blocks = (int32_t)std::max<int64_t>(1, std::min<int64_t>(M, 1024));
+  // scratch: per-source values (fwd) or [blocks][K] dbeta partials (bwd), plus
+  // one double for an absent total and the error word
+  const size_t need = sizeof(double) * (size_t)(std::max<int64_t>(M, (int64_t)blocks * c->K) + 2);
+  if (need > c->ll_scratch_bytes) {
+    if (c->d_ll_scratch) HIP_TRY(hipFree(c->d_ll_scratch));
+    c->d_ll_scratch = nullptr;
+    c->ll_scratch_bytes = 0;
+    HIP_TRY(hipMalloc((void**)&c->d_ll_scratch, need));
+    c->ll_scratch_bytes = need;
+  }
+  int32_t* d_err = (int32_t*)(c->d_ll_scratch + std::max<int64_t>(M, (int64_t)blocks * c->K) + 1);
+  HIP_TRY(launch_fill_u32((uint32_t*)d_err, 0u, 1, s));  // a kernel, not a memset node
+  LogLikArgs a;
+  a.pi = c->d_pi;
+  a.ld = c->ld;
+  a.K = c->K;
+  a.M = M;
+  a.nodes = nodes;
+  a.phi = phi;
+  a.phi_ld = phi_ld;
+  a.theta = theta ? theta : c->d_theta;
+  a.pair_ptr = pair_ptr;
+  a.pair_nbr = pair_nbr;
+  a.pair_y = pair_y;
+  a.epsilon = c->hp.epsilon;
+  a.per_source = per_source ? per_source : c->d_ll_scratch;
+  a.gout = gout;
+  a.grad_phi = grad_phi;
+  a.dbeta_partials = c->d_ll_scratch;
+  a.blocks = blocks;
+  a.N = c->N;
+  a.P = P;
+  a.error = d_err;
+  if (!bwd) {
+    HIP_TRY(launch_loglik(a, c->prec, false, s));
+    double* out = total ? total : c->d_ll_scratch + std::max<int64_t>(M, (int64_t)blocks * c->K);
+    HIP_TRY(launch_sum(a.per_source, M, out, s));
+  } else {
+    if (!grad_phi) {
+      set_error("ammsb_loglik_bwd: grad_phi is required");
+      return AMMSB_ERR_ARG;
+    }
+    HIP_TRY(launch_loglik(a, c->prec, true, s));
+    if (grad_theta)
+      HIP_TRY(launch_theta_grad(c->d_ll_scratch, blocks, c->K, a.theta, gout, grad_theta, s));
+  }
+  int32_t herr = 0;
+  HIP_TRY(hipMemcpyAsync(&herr, d_err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (herr) {
+    set_error("ammsb_loglik: neighbour id outside [0, N)");
+    return AMMSB_ERR_ARG;
+  }
+  return AMMSB_OK;
+}
+}  // namespace
+
+int ammsb_loglik_fwd(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi,
+                     int64_t phi_ld, const double* theta, const int64_t* pair_ptr,
+                     const int32_t* pair_nbr, const uint8_t* pair_y, int64_t P, double* per_source,
+                     double* total, void* stream) {
+  return loglik_common(c, M, nodes, phi, phi_ld, theta, pair_ptr, pair_nbr, pair_y, P, false,
+                       per_source, total, nullptr, nullptr, nullptr, stream);
+}
+
+int ammsb_loglik_bwd(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi,
+                     int64_t phi_ld, const double* theta, const int64_t* pair_ptr,
+                     const int32_t* pair_nbr, const uint8_t* pair_y, int64_t P, const double* gout,
+                     void* grad_phi, double* grad_theta, void* stream) {
+  return loglik_common(c, M, nodes, phi, phi_ld, theta, pair_ptr, pair_nbr, pair_y, P, true,
+                       nullptr, nullptr, gout, grad_phi, grad_theta, stream);
 }
 
 int ammsb_device_ptrs(ammsb_ctx* c, void** pi, void** theta, void** beta) {

@@ -32,7 +32,8 @@ enum : int32_t {
   ERR_LINK_ROUNDS = 2,        // link sampler found only empty minibatches
   ERR_NEIGHBOR_ATTEMPTS = 4,  // neighbour sampler exceeded its budget
   ERR_MB_OVERFLOW = 8,        // minibatch larger than its buffer
-  ERR_BAD_STRATEGY = 16
+  ERR_BAD_STRATEGY = 16,
+  ERR_BAD_INPUT = 32          // caller-provided device ids out of range
 };
 
 // Read-only graph view on the device.
@@ -228,4 +229,38 @@ hipError_t launch_label_nbrs(DevGraph g, const int32_t* nodes, int32_t n_nodes, 
 namespace ammsb {
 // Contiguous read of `bytes` (a multiple of 16) starting at p.
 hipError_t launch_read_probe(const void* p, int64_t bytes, double* sink, hipStream_t s);
+}  // namespace ammsb
+
+namespace ammsb {
+// Minibatch log-likelihood and its adjoint (differentiation).
+struct LogLikArgs {
+  const void* pi;            // resident state [N][ld] (neighbour rows)
+  int64_t ld;
+  int32_t K;
+  int64_t M;                 // source nodes
+  const int32_t* nodes;      // [M] source node ids (a pair with nbr == node is skipped)
+  const void* phi;           // [M][phi_ld] caller's phi rows of the sources
+  int64_t phi_ld;
+  const double* theta;       // [K][2]
+  const int64_t* pair_ptr;   // [M+1]
+  const int32_t* pair_nbr;   // [P]
+  const uint8_t* pair_y;     // [P]
+  double epsilon;
+  double* per_source;        // fwd: [M] log-likelihood of each source's pairs
+  const double* gout;        // bwd: dOut/dL (one double on the device)
+  void* grad_phi;            // bwd: [M][phi_ld]
+  double* dbeta_partials;    // bwd: [blocks][K]
+  int32_t blocks;
+  int32_t N;                 // node ids must lie in [0, N)
+  int64_t P;                 // pairs (pair_ptr is clamped to [0, P])
+  int32_t* error;            // ERR_BAD_INPUT is or-ed in for out-of-range ids
+};
+hipError_t launch_loglik(const LogLikArgs& a, int precision, bool backward, hipStream_t s);
+// total = sum_s per_source[s] (fixed order)
+hipError_t launch_sum(const double* x, int64_t n, double* out, hipStream_t s);
+// grad_theta[k][0..1] = gout * dbeta_k * (-theta_k1, theta_k0) / (theta_k0+theta_k1)^2,
+// dbeta_k = sum over blocks of dbeta_partials
+hipError_t launch_theta_grad(const double* dbeta_partials, int32_t blocks, int32_t K,
+                             const double* theta, const double* gout, double* grad_theta,
+                             hipStream_t s);
 }  // namespace ammsb

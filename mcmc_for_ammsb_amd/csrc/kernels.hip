@@ -1411,3 +1411,246 @@ hipError_t launch_read_probe(const void* p, int64_t bytes, double* sink, hipStre
   return hipGetLastError();
 }
 }  // namespace ammsb
+
+// ===========================================================================
+// Differentiation: the minibatch log-likelihood
+//   L = sum_s sum_{p in pairs(s)} log p(y_p | pi_s, pi_{b_p}, beta(theta))
+//   p(y) = sum_k pi_sk (pi_bk f_k + e),  f_k = beta_k - eps (y=1) / eps - beta_k
+//          (y=0), e = eps / 1 - eps  -- the likelihood of update_phi
+//          (mcmc_sampler_stochastic.cc:285-292) and of update_beta (:220-233)
+//   pi_s = phi_s / sum(phi_s) from the caller's rows, pi_b the resident rows
+//   (constants, as in update_phi), beta_k = theta_k1 / (theta_k0 + theta_k1).
+// Adjoint (recomputes every x_bk and Z instead of storing them):
+//   dL/dphi_sk = sum_p (x_pk / Z_p - 1) / sum(phi_s)   (update_phi's grads)
+//   dL/dbeta_k = sum_p (2y_p - 1) pi_sk pi_bk / Z_p
+//   dL/dtheta_k0 = -theta_k1/S^2 dL/dbeta_k, dL/dtheta_k1 = theta_k0/S^2 dL/dbeta_k
+//   (equal to update_beta's G_k0, G_k1, mcmc_sampler_stochastic.cc:234-239)
+namespace ammsb {
+namespace {
+
+template <typename F, int LPR, int CH, int WAVES, bool BWD>
+__global__ __launch_bounds__(WAVES * 64) void k_loglik(LogLikArgs a) {
+  constexpr int V = VT<F>::V;
+  constexpr int EPL = V * CH;
+  constexpr int RPW = 64 / LPR;
+  constexpr int STRIDE = WAVES * RPW;
+  __shared__ double s_red[(WAVES > 1 ? WAVES - 1 : 1) * EPL * LPR];
+  __shared__ double s_ll[WAVES];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int g = lane / LPR, lg = lane % LPR;
+  const int K = a.K;
+  const F* pi = (const F*)a.pi;
+  const F eps = (F)a.epsilon;
+  F f1[EPL];
+  double dbeta[EPL];
+#pragma unroll
+  for (int c = 0; c < CH; ++c)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int k = (c * LPR + lg) * V + v;
+      const int j = c * V + v;
+      double bk = 0.0;
+      if (k < K) bk = a.theta[2 * k + 1] / (a.theta[2 * k] + a.theta[2 * k + 1]);
+      f1[j] = (F)bk - eps;
+      dbeta[j] = 0.0;
+    }
+  const F e1 = eps, e0 = (F)1 - eps;
+  for (int64_t s = blockIdx.x; s < a.M; s += gridDim.x) {
+    const int32_t node = a.nodes[s];
+    const F* phis = (const F*)a.phi + s * a.phi_ld;
+    F pis[EPL];
+    double sp = 0.0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const int k = (c * LPR + lg) * V + v;
+        pis[c * V + v] = k < K ? phis[k] : (F)0;
+        sp += (double)pis[c * V + v];
+      }
+    sp = group_sum<LPR>(sp);  // sum(phi_s) in double
+    if (RPW > 1) sp = read_lane(sp, 0);
+#pragma unroll
+    for (int j = 0; j < EPL; ++j) pis[j] = (F)((double)pis[j] / sp);
+    F acc[EPL];
+#pragma unroll
+    for (int j = 0; j < EPL; ++j) acc[j] = (F)0;
+    double ll = 0.0;
+    // pair offsets clamped to [0, P]; neighbour ids outside [0, N) are
+    // skipped and reported (caller-provided device arrays)
+    const int64_t p0 = min(max(a.pair_ptr[s], (int64_t)0), a.P);
+    const int64_t p1 = min(max(a.pair_ptr[s + 1], p0), a.P);
+    for (int64_t p = p0 + w * RPW + g; p < p1; p += STRIDE) {
+      int32_t b = a.pair_nbr[p];
+      int y = a.pair_y[p] ? 1 : 0;
+      if (LPR == 64) {
+        b = __builtin_amdgcn_readfirstlane(b);
+        y = __builtin_amdgcn_readfirstlane(y);
+      }
+      if (b < 0 || b >= a.N) {
+        if (lg == 0) atomicOr(a.error, ERR_BAD_INPUT);
+        continue;
+      }
+      if (b == node) continue;  // mcmc_sampler_stochastic.cc:275-277
+      F rb[EPL];
+      load_row<F, LPR, CH>(pi + (int64_t)b * a.ld, K, lg, rb);
+      F z = (F)0;
+#pragma unroll
+      for (int j = 0; j < EPL; ++j)
+        z = fma(pis[j], y ? fma(rb[j], f1[j], e1) : fma(-rb[j], f1[j], e0), z);
+      z = group_sum<LPR>(z);
+      if (!BWD) {
+        if (lg == 0) ll += log((double)z);
+      } else {
+        const F rz = recip(z);
+        const double sg = y ? 1.0 : -1.0;
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) {
+          const F x = y ? fma(rb[j], f1[j], e1) : fma(-rb[j], f1[j], e0);  // recomputed
+          acc[j] += fma(x, rz, (F)-1);
+          dbeta[j] += sg * (double)(pis[j] * rb[j] * rz);
+        }
+      }
+    }
+    if (!BWD) {
+      // lanes with lg != 0 hold zeros
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) ll += __shfl_xor(ll, off, 64);
+      if (lane == 0) s_ll[w] = ll;
+      __syncthreads();
+      if (tid == 0) {
+        double t = 0.0;
+        for (int q = 0; q < WAVES; ++q) t += s_ll[q];
+        a.per_source[s] = t;
+      }
+      __syncthreads();
+    } else {
+      if (RPW > 1) {
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) acc[j] = cross_group_sum<LPR>(acc[j]);
+      }
+      if (WAVES > 1) {
+        if (w > 0 && g == 0) {
+#pragma unroll
+          for (int j = 0; j < EPL; ++j) s_red[((w - 1) * EPL + j) * LPR + lg] = (double)acc[j];
+        }
+        __syncthreads();
+        if (w == 0) {
+#pragma unroll
+          for (int q = 1; q < WAVES; ++q)
+#pragma unroll
+            for (int j = 0; j < EPL; ++j) acc[j] += (F)s_red[((q - 1) * EPL + j) * LPR + lg];
+        }
+        __syncthreads();
+      }
+      if (w == 0 && g == 0) {
+        const double go = a.gout[0];
+        F* gp = (F*)a.grad_phi + s * a.phi_ld;
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            const int k = (c * LPR + lg) * V + v;
+            if (k < K) gp[k] = (F)(go * (double)acc[c * V + v] / sp);
+          }
+      }
+    }
+  }
+  if (BWD) {
+    if (RPW > 1) {
+#pragma unroll
+      for (int j = 0; j < EPL; ++j) dbeta[j] = cross_group_sum<LPR>(dbeta[j]);
+    }
+    if (WAVES > 1) {
+      if (w > 0 && g == 0) {
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) s_red[((w - 1) * EPL + j) * LPR + lg] = dbeta[j];
+      }
+      __syncthreads();
+      if (w == 0) {
+#pragma unroll
+        for (int q = 1; q < WAVES; ++q)
+#pragma unroll
+          for (int j = 0; j < EPL; ++j) dbeta[j] += s_red[((q - 1) * EPL + j) * LPR + lg];
+      }
+    }
+    if (w == 0 && g == 0) {
+      double* out = a.dbeta_partials + (int64_t)blockIdx.x * K;
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const int k = (c * LPR + lg) * V + v;
+          if (k < K) out[k] = dbeta[c * V + v];
+        }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sum(const double* x, int64_t n, double* out) {
+  __shared__ double s[256];
+  double v = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) v += x[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 128; off >= 1; off >>= 1) {
+    if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
+__global__ __launch_bounds__(256) void k_theta_grad(const double* part, int blocks, int K,
+                                                    const double* theta, const double* gout,
+                                                    double* grad) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= K) return;
+  double d = 0.0;
+  for (int b = 0; b < blocks; ++b) d += part[(int64_t)b * K + k];
+  const double t0 = theta[2 * k], t1 = theta[2 * k + 1];
+  const double S = t0 + t1;
+  const double go = gout[0];
+  grad[2 * k] = go * d * (-t1 / (S * S));
+  grad[2 * k + 1] = go * d * (t0 / (S * S));
+}
+
+}  // namespace
+
+hipError_t launch_loglik(const LogLikArgs& a, int precision, bool backward, hipStream_t s) {
+  int lpr, ch;
+  if (!row_config(a.K, precision, &lpr, &ch)) return hipErrorInvalidValue;
+  if (a.M <= 0) return hipSuccess;
+  const dim3 grid(a.blocks), block(4 * 64);
+#define X(L, C)                                                                                 \
+  if (lpr == L && ch == C) {                                                                    \
+    if (precision == 0) {                                                                       \
+      if (backward)                                                                             \
+        hipLaunchKernelGGL((k_loglik<double, L, C, 4, true>), grid, block, 0, s, a);            \
+      else                                                                                      \
+        hipLaunchKernelGGL((k_loglik<double, L, C, 4, false>), grid, block, 0, s, a);           \
+    } else {                                                                                    \
+      if (backward)                                                                             \
+        hipLaunchKernelGGL((k_loglik<float, L, C, 4, true>), grid, block, 0, s, a);             \
+      else                                                                                      \
+        hipLaunchKernelGGL((k_loglik<float, L, C, 4, false>), grid, block, 0, s, a);            \
+    }                                                                                           \
+    return hipGetLastError();                                                                   \
+  }
+  AMMSB_ROW_CONFIGS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_sum(const double* x, int64_t n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, x, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_theta_grad(const double* part, int32_t blocks, int32_t K, const double* theta,
+                             const double* gout, double* grad, hipStream_t s) {
+  hipLaunchKernelGGL(k_theta_grad, dim3((K + 255) / 256), dim3(256), 0, s, part, blocks, K, theta,
+                     gout, grad);
+  return hipGetLastError();
+}
+
+}  // namespace ammsb

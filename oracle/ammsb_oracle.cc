@@ -20,6 +20,8 @@
 //                    Learner::cal_edge_likelihood, learner.h:100-139
 //   orc_eps_t        get_eps_t, mcmc_sampler_stochastic.h:112-115
 //   orc_run          MCMCSamplerStochastic::run, mcmc_sampler_stochastic.cc:93-183
+//   orc_phi_grads / orc_beta_grads   the closed-form gradients of update_phi
+//                    (distr.cc:1165-1197) and update_beta (:202-240)
 //   orc_sample_minibatch   Network::stratified_random_node_sampling{,_links,
 //                    _nonlinks}, network.cc:274-421, with the counter-based
 //                    draw addressing of this project (see below)
@@ -695,6 +697,80 @@ double orc_eps_t(const double* hyper, int64_t step, int32_t precision) {
   const Hyper hp = hyper_from(hyper);
   return precision == 0 ? (double)eps_t_of<double>(hp, (size_t)step)
                         : (double)eps_t_of<float>(hp, (size_t)step);
+}
+
+// The reference's closed-form gradients, before any step is taken:
+// update_phi_node's grads (distr.cc:1165-1197) for one node and its
+// neighbours, and update_beta's grads (mcmc_sampler_stochastic.cc:202-240)
+// for a list of edges.  Both read the handle's state.
+void orc_phi_grads(void* hv, int32_t node, const int32_t* nbrs, int64_t nnb, double* out) {
+  Handle* h = (Handle*)hv;
+  auto run = [&](auto& S) {
+    using Float = typename std::decay_t<decltype(S.pi)>::value_type;
+    const int64_t K = S.K;
+    const Float* pi_node = S.row(node);
+    const Float epsilon = (Float)S.hp.epsilon;
+    Float phi_i_sum = pi_node[K];
+    std::vector<Float> grads(K, 0.0), probs(K);
+    for (int64_t ix = 0; ix < nnb; ++ix) {
+      const int32_t neighbor = nbrs[ix] & 0x7fffffff;
+      if (node == neighbor) continue;
+      const int y_ab = S.g->linked(std::min(node, neighbor), std::max(node, neighbor)) ? 1 : 0;
+      const Float* pi_b = S.row(neighbor);
+      Float e = (y_ab == 1) ? epsilon : 1.0 - epsilon;
+      for (int64_t k = 0; k < K; ++k) {
+        Float f = (y_ab == 1) ? (S.beta[k] - epsilon) : (epsilon - S.beta[k]);
+        probs[k] = pi_node[k] * (pi_b[k] * f + e);
+      }
+      Float prob_sum = std::accumulate(probs.begin(), probs.end(), static_cast<Float>(0));
+      for (int64_t k = 0; k < K; ++k) grads[k] += ((probs[k] / prob_sum) / pi_node[k] - 1.0) / phi_i_sum;
+    }
+    for (int64_t k = 0; k < K; ++k) out[k] = grads[k];
+  };
+  if (h->prec == 0)
+    run(h->s64);
+  else
+    run(h->s32);
+}
+
+void orc_beta_grads(void* hv, const int32_t* edges, const uint8_t* ys, int64_t E, double* out) {
+  Handle* h = (Handle*)hv;
+  auto run = [&](auto& S) {
+    using Float = typename std::decay_t<decltype(S.pi)>::value_type;
+    const int64_t K = S.K;
+    const Float epsilon = (Float)S.hp.epsilon;
+    std::vector<std::vector<Float>> grads(K, std::vector<Float>(2, (Float)0.0));
+    std::vector<Float> probs(K), theta_sum(K);
+    for (int64_t k = 0; k < K; ++k)
+      theta_sum[k] = std::accumulate(S.theta[k].begin(), S.theta[k].end(), static_cast<Float>(0));
+    for (int64_t e = 0; e < E; ++e) {
+      const int y = ys[e] ? 1 : 0;
+      const Float* pi_i = S.row(edges[2 * e]);
+      const Float* pi_j = S.row(edges[2 * e + 1]);
+      Float pi_sum = (Float)0.0;
+      for (int64_t k = 0; k < K; ++k) {
+        pi_sum += pi_i[k] * pi_j[k];
+        Float f = pi_i[k] * pi_j[k];
+        probs[k] = (y == 1) ? S.beta[k] * f : ((Float)1.0 - S.beta[k]) * f;
+      }
+      Float prob_0 = ((y == 1) ? epsilon : ((Float)1.0 - epsilon)) * ((Float)1.0 - pi_sum);
+      Float prob_sum = std::accumulate(probs.begin(), probs.end(), static_cast<Float>(0)) + prob_0;
+      for (int64_t k = 0; k < K; ++k) {
+        Float f = probs[k] / prob_sum;
+        Float one_over_theta_sum = (Float)1.0 / theta_sum[k];
+        grads[k][0] += f * (((Float)1.0 - y) / S.theta[k][0] - one_over_theta_sum);
+        grads[k][1] += f * (y / S.theta[k][1] - one_over_theta_sum);
+      }
+    }
+    for (int64_t k = 0; k < K; ++k) {
+      out[2 * k] = grads[k][0];
+      out[2 * k + 1] = grads[k][1];
+    }
+  };
+  if (h->prec == 0)
+    run(h->s64);
+  else
+    run(h->s32);
 }
 
 // Sampler replicas.  edges_out needs room for 2*cap ints.

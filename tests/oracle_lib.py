@@ -39,6 +39,8 @@ def _load():
         "orc_phi_noise": (None, [c_i32, c_u64, c_i32, c_i64, c_i64, c_vp]),
         "orc_theta_noise": (None, [c_u64, c_i64, c_i64, P(c_dbl)]),
         "orc_log_gamma": (c_dbl, [c_u64, c_dbl, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+        "orc_phi_grads": (None, [c_vp, c_i32, P(c_i32), c_i64, P(c_dbl)]),
+        "orc_beta_grads": (None, [c_vp, P(c_i32), P(ctypes.c_uint8), c_i64, P(c_dbl)]),
         "orc_run": (c_i32, [c_vp, c_i64, c_i32, c_i64, c_i64, c_i64, c_u64, c_i32, P(c_dbl), P(c_i64),
                             c_i64, P(c_i64), P(c_dbl)]),
     }
@@ -157,6 +159,19 @@ class Oracle:
     def theta_noise(self, step, seed):
         out = np.zeros((self.K, 2))
         lib.orc_theta_noise(seed, step, self.K, _p(out, c_dbl))
+        return out
+
+    def phi_grads(self, node, nbrs):
+        nbrs = np.ascontiguousarray(nbrs, dtype=np.int32)
+        out = np.zeros(self.K)
+        lib.orc_phi_grads(self.h, int(node), _p(nbrs, c_i32), len(nbrs), _p(out, c_dbl))
+        return out
+
+    def beta_grads(self, edges, y):
+        edges = np.ascontiguousarray(edges, dtype=np.int32)
+        y = np.ascontiguousarray(y, dtype=np.uint8)
+        out = np.zeros((self.K, 2))
+        lib.orc_beta_grads(self.h, _p(edges, c_i32), _p(y, ctypes.c_uint8), len(y), _p(out, c_dbl))
         return out
 
     def run(self, iters, strategy, m, max_source, interval, seed, threads=1, cap=100000):

@@ -1,0 +1,150 @@
+"""Differentiation: forward minibatch log-likelihood and its adjoint (GPU).
+
+Pins:
+  * forward value: brute force over all (z_ab, z_ba) community pairs of the
+    a-MMSB likelihood (the model's definition), on a few sources
+  * dL/dphi: the reference's closed-form update_phi gradient (oracle,
+    update_phi_node's grads) to 1e-12; central finite differences
+    (h = 1e-6, FP64) on random coordinates
+  * dL/dtheta: the reference's update_beta gradient (oracle) to 1e-12;
+    central finite differences on every coordinate
+  * FP32 precision mode against FP64 to 1e-4
+"""
+import numpy as np
+import pytest
+
+import mcmc_for_ammsb_amd as M
+from tests.oracle_lib import Oracle
+
+torch = pytest.importorskip("torch")
+import mcmc_for_ammsb_amd.autograd  # noqa: E402,F401  (M.autograd)
+
+pytestmark = pytest.mark.gpu
+
+N, K, MSRC, NNL = 20000, 48, 40, 24
+
+
+@pytest.fixture(scope="module")
+def setup():
+    g = M.generate("community", N=N, K=K, mean_degree=20.0, seed=9)
+    s = M.make_split(g, 0.1, 42)
+    rng = np.random.default_rng(3)
+    phi = rng.gamma(1.0, 1.0, size=(N, K))
+    pi = np.concatenate([phi / phi.sum(1, keepdims=True), phi.sum(1, keepdims=True)], 1)
+    theta = rng.gamma(2.0, 0.5, size=(K, 2))
+    nodes = np.sort(rng.choice(N, size=MSRC, replace=False)).astype(np.int32)
+    ptr, nbr, y = M.autograd.make_pairs(g, s, nodes, NNL, seed=1)
+    return g, s, pi, theta, nodes, ptr, nbr, y
+
+
+def _dev(x, dt=None):
+    t = torch.from_numpy(np.ascontiguousarray(x))
+    return (t.to(dt) if dt is not None else t).cuda()
+
+
+def _sampler(g, s, pi, theta, precision="fp64"):
+    smp = M.MCMCSamplerStochastic(M.Options(K=K, num_node_sample=10, precision=precision), g, s)
+    smp.set_state(pi.astype(np.float64 if precision == "fp64" else np.float32), theta)
+    return smp
+
+
+def _inputs(pi, theta, nodes, ptr, nbr, y, dt=torch.float64):
+    phi_src = pi[nodes, :K] * pi[nodes, K:K + 1]
+    return (_dev(phi_src, dt).requires_grad_(True), _dev(theta).requires_grad_(True), _dev(nodes),
+            _dev(ptr), _dev(nbr), _dev(y))
+
+
+def test_forward_matches_brute_force(setup):
+    g, s, pi, theta, nodes, ptr, nbr, y = setup
+    smp = _sampler(g, s, pi, theta)
+    phi_t, th_t, n_t, p_t, b_t, y_t = _inputs(pi, theta, nodes, ptr, nbr, y)
+    per = M.autograd.loglik_per_source(smp, phi_t, th_t, n_t, p_t, b_t, y_t).cpu().numpy()
+    beta = theta[:, 1] / theta.sum(1)
+    eps = 1e-7
+    for q in range(4):
+        ref = 0.0
+        pa = pi[nodes[q], :K]
+        for p in range(ptr[q], ptr[q + 1]):
+            pb = pi[nbr[p], :K]
+            outer = np.outer(pa, pb)  # z_ab = k, z_ba = l
+            p1 = (np.diag(outer) * beta).sum() + eps * (outer.sum() - np.trace(outer))
+            ref += np.log(p1 if y[p] else 1.0 - p1)
+        assert per[q] == pytest.approx(ref, rel=1e-12)
+    total = M.autograd.loglik(smp, phi_t, th_t, n_t, p_t, b_t, y_t)
+    assert total.item() == pytest.approx(per.sum(), rel=1e-13)
+
+
+def test_adjoint_matches_reference_closed_form(setup):
+    g, s, pi, theta, nodes, ptr, nbr, y = setup
+    smp = _sampler(g, s, pi, theta)
+    orc = Oracle(g, s, K, num_node_sample=10)
+    orc.set_state(pi, theta)
+    phi_t, th_t, n_t, p_t, b_t, y_t = _inputs(pi, theta, nodes, ptr, nbr, y)
+    L = M.autograd.loglik(smp, phi_t, th_t, n_t, p_t, b_t, y_t)
+    L.backward()
+    gphi = phi_t.grad.cpu().numpy()
+    gth = th_t.grad.cpu().numpy()
+    for q in range(MSRC):
+        ref = orc.phi_grads(nodes[q], nbr[ptr[q]:ptr[q + 1]])
+        assert np.allclose(gphi[q], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max()), q
+    src = np.repeat(nodes, np.diff(ptr))
+    edges = np.stack([src, nbr], 1)
+    ref = orc.beta_grads(edges, y)
+    assert np.allclose(gth, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_adjoint_matches_finite_differences(setup):
+    g, s, pi, theta, nodes, ptr, nbr, y = setup
+    smp = _sampler(g, s, pi, theta)
+    phi_t, th_t, n_t, p_t, b_t, y_t = _inputs(pi, theta, nodes, ptr, nbr, y)
+    L = M.autograd.loglik(smp, phi_t, th_t, n_t, p_t, b_t, y_t)
+    L.backward()
+    gphi = phi_t.grad.cpu().numpy()
+    gth = th_t.grad.cpu().numpy()
+    h = 1e-6
+    rng = np.random.default_rng(5)
+    base_phi = phi_t.detach().clone()
+    for _ in range(200):
+        q, k = int(rng.integers(MSRC)), int(rng.integers(K))
+        vals = []
+        for sgn in (1, -1):
+            ph = base_phi.clone()
+            ph[q, k] += sgn * h
+            per = M.autograd.loglik_per_source(smp, ph, th_t.detach(), n_t, p_t, b_t, y_t)
+            vals.append(per[q].item())  # only source q depends on phi_q
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert fd == pytest.approx(gphi[q, k], rel=1e-5, abs=1e-6), (q, k)
+    base_th = th_t.detach().clone()
+    for k in range(K):
+        for i in range(2):
+            vals = []
+            for sgn in (1, -1):
+                th = base_th.clone()
+                th[k, i] += sgn * h
+                vals.append(M.autograd.loglik_per_source(smp, base_phi, th, n_t, p_t, b_t, y_t).sum().item())
+            fd = (vals[0] - vals[1]) / (2 * h)
+            assert fd == pytest.approx(gth[k, i], rel=1e-5, abs=1e-4), (k, i)
+
+
+def test_fp32_against_fp64(setup):
+    g, s, pi, theta, nodes, ptr, nbr, y = setup
+    out = {}
+    for prec, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        smp = _sampler(g, s, pi, theta, prec)
+        phi_t, th_t, n_t, p_t, b_t, y_t = _inputs(pi, theta, nodes, ptr, nbr, y, dt)
+        L = M.autograd.loglik(smp, phi_t, th_t, n_t, p_t, b_t, y_t)
+        L.backward()
+        out[prec] = (L.item(), phi_t.grad.double().cpu().numpy(), th_t.grad.cpu().numpy())
+    assert out["fp32"][0] == pytest.approx(out["fp64"][0], rel=1e-5)
+    assert np.allclose(out["fp32"][1], out["fp64"][1], rtol=1e-3, atol=1e-4 * np.abs(out["fp64"][1]).max())
+    assert np.allclose(out["fp32"][2], out["fp64"][2], rtol=1e-3, atol=1e-4 * np.abs(out["fp64"][2]).max())
+
+
+def test_bad_neighbour_id_is_an_error(setup):
+    g, s, pi, theta, nodes, ptr, nbr, y = setup
+    smp = _sampler(g, s, pi, theta)
+    bad = nbr.copy()
+    bad[3] = N + 7
+    phi_t, th_t, n_t, p_t, b_t, y_t = _inputs(pi, theta, nodes, ptr, bad, y)
+    with pytest.raises(M.AmmsbError):
+        M.autograd.loglik(smp, phi_t, th_t, n_t, p_t, b_t, y_t)
