@@ -193,6 +193,126 @@ def run_ttt(args):
     return 0
 
 
+def run_diff(args):
+    """Differentiation workload: forward minibatch log-likelihood and its
+    adjoint over 8,192 source nodes' training edges + 256 non-edges each, on
+    an N=1M, K=2048 community graph (mean degree 50), FP64 and FP32; central
+    finite differences (h=1e-6, FP64) on 1,000 random coordinates (800 of
+    phi, 200 of theta)."""
+    import torch
+    import mcmc_for_ammsb_amd as M
+    import mcmc_for_ammsb_amd.autograd as A
+    from mcmc_for_ammsb_amd import _lib
+    import ctypes
+    N, K, deg, MS, NNL = 1_000_000, 2048, 50.0, 8192, 256
+    t = time.time()
+    g = M.generate("community", N=N, K=K, mean_degree=deg, seed=args.seed)
+    s = M.make_split(g, 0.1, 42)
+    rng = np.random.default_rng(7)
+    nodes = np.sort(rng.choice(N, size=MS, replace=False)).astype(np.int32)
+    ptr, nbr, y = A.make_pairs(g, s, nodes, NNL, seed=1)
+    log(f"graph N={g.N} E={g.num_edges}, {MS} sources, {len(nbr)} pairs ({time.time() - t:.1f}s)")
+    dev = torch.device("cuda:0")
+    n_t = torch.from_numpy(nodes).to(dev)
+    p_t = torch.from_numpy(ptr).to(dev)
+    b_t = torch.from_numpy(nbr).to(dev)
+    y_t = torch.from_numpy(y).to(dev)
+    out = {"metric": "minibatch log-likelihood forward + adjoint, pair*K/s",
+           "workload": f"N={N} K={K} community graph mean degree {deg:g}, {MS} sources, "
+                       f"all training edges + {NNL} non-edges each ({len(nbr)} pairs)",
+           "pairs": int(len(nbr)), "sources": MS, "results": {}}
+    for prec in ("fp64", "fp32"):
+        F = 8 if prec == "fp64" else 4
+        opts = M.Options(K=K, num_node_sample=128, precision=prec, random_seed=42)
+        smp = M.MCMCSamplerStochastic(opts, g, s)
+        smp.init(42)
+        rows = smp.get_rows(nodes)
+        theta = np.empty((K, 2))
+        _lib.check(_lib.lib.ammsb_get_state(smp._h, None, _lib.ptr(theta, ctypes.c_double), None))
+        dt = torch.float64 if prec == "fp64" else torch.float32
+        phi = torch.from_numpy(np.ascontiguousarray(rows[:, :K] * rows[:, K:K + 1])).to(dev, dt)
+        th = torch.from_numpy(theta).to(dev)
+        for _ in range(2):
+            A.loglik_per_source(smp, phi, th, n_t, p_t, b_t, y_t)
+            A.loglik_grad(smp, phi, th, n_t, p_t, b_t, y_t)
+        torch.cuda.synchronize()
+        reps = args.diff_reps
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record()
+        for _ in range(reps):
+            per = A.loglik_per_source(smp, phi, th, n_t, p_t, b_t, y_t)
+        ev[1].record()
+        ev[2].record()
+        for _ in range(reps):
+            gphi, gth = A.loglik_grad(smp, phi, th, n_t, p_t, b_t, y_t)
+        ev[3].record()
+        torch.cuda.synchronize()
+        t_f = ev[0].elapsed_time(ev[1]) / 1e3 / reps
+        t_b = ev[2].elapsed_time(ev[3]) / 1e3 / reps
+        P = len(nbr)
+        fwd_bytes = P * K * F + MS * K * F + P * 5 + (MS + 1) * 8 + MS * 8
+        bwd_bytes = fwd_bytes - MS * 8 + MS * K * F + min(MS, 1024) * K * 8
+        r = {"fwd_ms": 1e3 * t_f, "bwd_ms": 1e3 * t_b,
+             "fwd_pair_k_per_s": P * K / t_f, "bwd_pair_k_per_s": P * K / t_b,
+             "fwd_GBs": fwd_bytes / t_f / 1e9, "bwd_GBs": bwd_bytes / t_b / 1e9,
+             "fwd_frac_of_hbm_peak": fwd_bytes / t_f / 1e9 / HBM_PEAK_GBS,
+             "bwd_frac_of_hbm_peak": bwd_bytes / t_b / 1e9 / HBM_PEAK_GBS,
+             "loglik": float(per.sum().item())}
+        if prec == "fp64":
+            r.update(finite_differences(A, smp, phi, th, n_t, p_t, b_t, y_t, gphi, gth, ptr))
+            ref64 = (per.clone(), gphi.clone(), gth.clone())
+        else:
+            r["fp32_vs_fp64"] = {
+                "loglik_rel": float(abs(per.sum().item() / ref64[0].sum().item() - 1)),
+                "grad_phi_max_abs_over_max": float((gphi.double() - ref64[1]).abs().max()
+                                                   / ref64[1].abs().max()),
+                "grad_theta_max_abs_over_max": float((gth - ref64[2]).abs().max()
+                                                     / ref64[2].abs().max())}
+        out["results"][prec] = r
+        log(f"{prec}: fwd {1e3 * t_f:.3f} ms ({r['fwd_GBs']:.0f} GB/s), "
+            f"bwd {1e3 * t_b:.3f} ms ({r['bwd_GBs']:.0f} GB/s)")
+        del phi, gphi, gth, per
+        smp.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def finite_differences(A, smp, phi, th, n_t, p_t, b_t, y_t, gphi, gth, ptr, h=1e-6):
+    """Central differences on 800 phi and 200 theta coordinates (FP64)."""
+    import torch
+    rng = np.random.default_rng(11)
+    MS, K = phi.shape
+    errs_phi, errs_th = [], []
+    gmax = float(gphi.abs().max())
+    for _ in range(800):
+        q, k = int(rng.integers(MS)), int(rng.integers(K))
+        lo, hi = int(ptr[q]), int(ptr[q + 1])
+        sub_p = torch.tensor([0, hi - lo], dtype=torch.int64, device=phi.device)
+        vals = []
+        for sg in (1, -1):
+            ph = phi[q:q + 1].clone()
+            ph[0, k] += sg * h
+            vals.append(A.loglik_per_source(smp, ph, th, n_t[q:q + 1], sub_p, b_t[lo:hi],
+                                            y_t[lo:hi])[0].item())
+        fd = (vals[0] - vals[1]) / (2 * h)
+        errs_phi.append(abs(fd - gphi[q, k].item()) / (abs(gphi[q, k].item()) + 1e-3 * gmax))
+    tmax = float(gth.abs().max())
+    for _ in range(200):
+        k, i = int(rng.integers(K)), int(rng.integers(2))
+        vals = []
+        for sg in (1, -1):
+            t2 = th.clone()
+            t2[k, i] += sg * h
+            vals.append(A.loglik_per_source(smp, phi, t2, n_t, p_t, b_t, y_t).sum().item())
+        fd = (vals[0] - vals[1]) / (2 * h)
+        errs_th.append(abs(fd - gth[k, i].item()) / (abs(gth[k, i].item()) + 1e-3 * tmax))
+    return {"fd_coordinates": 1000, "fd_h": h,
+            "fd_phi_max_rel_err": float(max(errs_phi)),
+            "fd_phi_median_rel_err": float(np.median(errs_phi)),
+            "fd_theta_max_rel_err": float(max(errs_th)),
+            "fd_theta_median_rel_err": float(np.median(errs_th))}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,7 +326,8 @@ def main():
     ap.add_argument("--phi-waves", type=int, default=0, help="update_phi wavefronts per node")
     ap.add_argument("--phi-unroll", type=int, default=0, help="update_phi rows in flight")
     ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "update_phi_traffic.json"))
-    ap.add_argument("--mode", default="throughput", choices=["throughput", "ttt"])
+    ap.add_argument("--mode", default="throughput", choices=["throughput", "ttt", "diff"])
+    ap.add_argument("--diff-reps", type=int, default=10)
     ap.add_argument("--ttt-steps", type=int, default=3000)
     ap.add_argument("--ttt-interval", type=int, default=100)
     args = ap.parse_args()
@@ -217,6 +338,8 @@ def main():
     rank = int(os.environ.get("RANK", "0"))
     if args.mode == "ttt":
         return run_ttt(args)
+    if args.mode == "diff":
+        return run_diff(args)
 
     import mcmc_for_ammsb_amd as M
     w = WORKLOADS[args.workload]

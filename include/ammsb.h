@@ -278,9 +278,16 @@ int ammsb_get_totals(ammsb_ctx *ctx, int64_t *nodes_updated,
 /* Launch-shape tuning (no counterpart in the reference, which runs on CPU):
  *   "phi_waves"  wavefronts per minibatch node in update_phi (2, 4, 8)
  *   "phi_unroll" neighbour rows in flight per row group (1, 2, 4)
+ *   "ll_fwd_pairs", "ll_bwd_pairs"  pairs per LDS exchange of the
+ *                K-chunked log-likelihood forward / adjoint kernels (rows
+ *                wider than 2 KB): 2, 4, 8
+ *   "ll_fwd_prefetch", "ll_bwd_prefetch"  1: the next pairs' rows are
+ *                loaded before the exchange
  * Set phi_waves first; the (waves, unroll) pair is validated when
- * phi_unroll is set.  Results do not depend on the shape beyond summation
- * order.  Defaults (4, 1). */
+ * phi_unroll is set.  Results do not depend on the shape beyond
+ * summation order (the log-likelihood shapes not even on that).  Defaults:
+ * update_phi (4, 1); log-likelihood FP64 (4,0) both ways, FP32 forward
+ * (8,0), adjoint (2,1). */
 int ammsb_set_tuning(ammsb_ctx *ctx, const char *name, int64_t value);
 int ammsb_get_tuning(ammsb_ctx *ctx, const char *name, int64_t *value);
 

@@ -90,33 +90,53 @@ def loglik_per_source(sampler, phi, theta, nodes, pair_ptr, pair_nbr, pair_y):
     return per
 
 
+def loglik_grad(sampler, phi, theta, nodes, pair_ptr, pair_nbr, pair_y, gout=None):
+    """The adjoint only (no autograd graph): returns (dL/dphi, dL/dtheta),
+    scaled by ``gout`` (a 0-d float64 GPU tensor, default 1)."""
+    if gout is None:
+        gout = torch.ones((), dtype=torch.float64, device=phi.device)
+    grad_phi = torch.empty_like(phi)
+    grad_theta = torch.empty_like(theta)
+    stream = torch.cuda.current_stream(phi.device).cuda_stream
+    _lib.check(_lib.lib.ammsb_loglik_bwd(
+        sampler._h, nodes.shape[0], nodes.data_ptr(), phi.data_ptr(), phi.stride(0),
+        theta.data_ptr(), pair_ptr.data_ptr(), pair_nbr.data_ptr(), pair_y.data_ptr(),
+        pair_nbr.shape[0], gout.data_ptr(), grad_phi.data_ptr(), grad_theta.data_ptr(), stream))
+    return grad_phi, grad_theta
+
+
 def make_pairs(graph, split, nodes, n_nonlinks, seed=0):
     """Pair lists of the differentiation workload: for each source node, all
-    its training edges (y = 1; held-out and test edges excluded) followed by
-    ``n_nonlinks`` distinct random non-edges (y = 0, not held-out/test, not
-    the node itself).  Returns numpy (pair_ptr int64 [M+1], pair_nbr int32,
-    pair_y uint8)."""
+    its training edges (y = 1; held-out and test edges excluded) in CSR order,
+    followed by ``n_nonlinks`` distinct random non-edges (y = 0, not
+    held-out/test, not the node itself) in draw order.  Returns numpy
+    (pair_ptr int64 [M+1], pair_nbr int32, pair_y uint8)."""
     rng = np.random.default_rng(seed)
-    ht = set()
-    for e in np.concatenate([split.held_out_edges, split.test_edges]):
-        ht.add((int(min(e)), int(max(e))))
+    N = np.int64(graph.N)
+    ex = np.concatenate([split.held_out_edges, split.test_edges]).astype(np.int64)
+    exkeys = np.unique(np.minimum(ex[:, 0], ex[:, 1]) * N + np.maximum(ex[:, 0], ex[:, 1]))
+
+    def excluded(s, v):
+        key = np.minimum(s, v) * N + np.maximum(s, v)
+        pos = np.minimum(np.searchsorted(exkeys, key), max(len(exkeys) - 1, 0))
+        return (exkeys[pos] == key) if len(exkeys) else np.zeros(len(v), dtype=bool)
+
     ptr = [0]
     nbr, ys = [], []
     for s in np.asarray(nodes, dtype=np.int64):
-        row = graph.cols[graph.row_ptr[s]:graph.row_ptr[s + 1]]
-        for v in row:
-            if (min(s, v), max(s, v)) not in ht:
-                nbr.append(v)
-                ys.append(1)
-        rowset = set(row.tolist())
-        got = set()
+        row = graph.cols[graph.row_ptr[s]:graph.row_ptr[s + 1]].astype(np.int64)
+        links = row[~excluded(s, row)]
+        got = np.empty(0, dtype=np.int64)
         while len(got) < n_nonlinks:
-            v = int(rng.integers(0, graph.N))
-            if v == s or v in rowset or v in got or (min(s, v), max(s, v)) in ht:
-                continue
-            got.add(v)
-            nbr.append(v)
-            ys.append(0)
-        ptr.append(len(nbr))
-    return (np.asarray(ptr, dtype=np.int64), np.asarray(nbr, dtype=np.int32),
-            np.asarray(ys, dtype=np.uint8))
+            v = rng.integers(0, graph.N, size=2 * n_nonlinks + 16)
+            pos = np.minimum(np.searchsorted(row, v), max(len(row) - 1, 0))
+            in_row = (row[pos] == v) if len(row) else np.zeros(len(v), dtype=bool)
+            v = v[(v != s) & ~in_row & ~excluded(s, v)]
+            v = np.concatenate([got, v])
+            _, first = np.unique(v, return_index=True)
+            got = v[np.sort(first)][:n_nonlinks]
+        nbr += [links, got]
+        ys += [np.ones(len(links), np.uint8), np.zeros(len(got), np.uint8)]
+        ptr.append(ptr[-1] + len(links) + len(got))
+    cat = (lambda xs, dt: np.concatenate(xs).astype(dt) if xs else np.empty(0, dt))
+    return np.asarray(ptr, dtype=np.int64), cat(nbr, np.int32), cat(ys, np.uint8)

@@ -94,6 +94,8 @@ struct ammsb_ctx {
   int64_t mbtable_cap = 0;
   int32_t mbtable_in_lds = 0;
   int32_t phi_waves = kPhiWavesDefault, phi_unroll = kPhiUnrollDefault;
+  // K-chunked log-likelihood launch shapes [0] forward, [1] adjoint
+  int32_t ll_pairs[2] = {0, 0}, ll_prefetch[2] = {0, 0};
   double* d_ll_scratch = nullptr;   // differentiation: per-source values / dbeta partials
   size_t ll_scratch_bytes = 0;
   int2* d_nbr_gtable = nullptr;
@@ -457,6 +459,13 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
   c->K = (int32_t)o.K;
   c->prec = o.precision;
   c->fsize = c->prec == AMMSB_FP64 ? 8 : 4;
+  // K-chunked log-likelihood shapes measured at N=1M, K=2048 (2.5M pairs):
+  // FP64 (4 pairs, no prefetch) both ways; FP32 forward (8, no prefetch),
+  // adjoint (2, prefetch), whose registers leave 4 wavefronts per SIMD
+  c->ll_pairs[0] = c->prec == AMMSB_FP64 ? 4 : 8;
+  c->ll_prefetch[0] = 0;
+  c->ll_pairs[1] = c->prec == AMMSB_FP64 ? 4 : 2;
+  c->ll_prefetch[1] = c->prec == AMMSB_FP64 ? 0 : 1;
   int lpr, ch;
   if (!row_config(c->K, c->prec, &lpr, &ch)) {
     set_error("ammsb_create: K too large for the device kernels (max 2048 FP64 / 4096 FP32)");
@@ -1151,6 +1160,20 @@ int ammsb_get_totals(ammsb_ctx* c, int64_t* nodes_updated, int64_t* minibatch_ed
 int ammsb_set_tuning(ammsb_ctx* c, const char* name, int64_t value) {
   if (!c || !name) return AMMSB_ERR_ARG;
   const std::string n(name);
+  if (n == "ll_fwd_pairs" || n == "ll_fwd_prefetch" || n == "ll_bwd_pairs" ||
+      n == "ll_bwd_prefetch") {
+    const int d = n.compare(0, 6, "ll_bwd") == 0 ? 1 : 0;
+    const bool is_pairs = n.size() > 6 && n.compare(n.size() - 5, 5, "pairs") == 0;
+    const int pairs = is_pairs ? (int)value : c->ll_pairs[d];
+    const int pf = is_pairs ? c->ll_prefetch[d] : (int)value;
+    if (!loglik_kc_shape_supported(pairs, pf)) {
+      set_error("ammsb_set_tuning: log-likelihood shape (pairs, prefetch) not available");
+      return AMMSB_ERR_ARG;
+    }
+    c->ll_pairs[d] = pairs;
+    c->ll_prefetch[d] = pf;
+    return AMMSB_OK;
+  }
   int waves = c->phi_waves, unroll = c->phi_unroll;
   if (n == "phi_waves")
     waves = (int)value;
@@ -1178,6 +1201,14 @@ int ammsb_get_tuning(ammsb_ctx* c, const char* name, int64_t* value) {
     *value = c->phi_waves;
   else if (n == "phi_unroll")
     *value = c->phi_unroll;
+  else if (n == "ll_fwd_pairs")
+    *value = c->ll_pairs[0];
+  else if (n == "ll_fwd_prefetch")
+    *value = c->ll_prefetch[0];
+  else if (n == "ll_bwd_pairs")
+    *value = c->ll_pairs[1];
+  else if (n == "ll_bwd_prefetch")
+    *value = c->ll_prefetch[1];
   else {
     set_error("ammsb_get_tuning: unknown parameter " + n);
     return AMMSB_ERR_ARG;
@@ -1259,7 +1290,7 @@ int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi
   a.P = P;
   a.error = d_err;
   if (!bwd) {
-    HIP_TRY(launch_loglik(a, c->prec, false, s));
+    HIP_TRY(launch_loglik(a, c->prec, false, s, c->ll_pairs[0], c->ll_prefetch[0]));
     double* out = total ? total : c->d_ll_scratch + std::max<int64_t>(M, (int64_t)blocks * c->K);
     HIP_TRY(launch_sum(a.per_source, M, out, s));
   } else {
@@ -1267,7 +1298,7 @@ int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi
       set_error("ammsb_loglik_bwd: grad_phi is required");
       return AMMSB_ERR_ARG;
     }
-    HIP_TRY(launch_loglik(a, c->prec, true, s));
+    HIP_TRY(launch_loglik(a, c->prec, true, s, c->ll_pairs[1], c->ll_prefetch[1]));
     if (grad_theta)
       HIP_TRY(launch_theta_grad(c->d_ll_scratch, blocks, c->K, a.theta, gout, grad_theta, s));
   }

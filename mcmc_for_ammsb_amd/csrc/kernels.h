@@ -255,7 +255,15 @@ struct LogLikArgs {
   int64_t P;                 // pairs (pair_ptr is clamped to [0, P])
   int32_t* error;            // ERR_BAD_INPUT is or-ed in for out-of-range ids
 };
-hipError_t launch_loglik(const LogLikArgs& a, int precision, bool backward, hipStream_t s);
+// pairs/prefetch: launch shape of the K-chunked kernel (pairs per LDS
+// exchange, next pairs' loads issued before the exchange); ignored when one
+// wavefront holds a row
+hipError_t launch_loglik(const LogLikArgs& a, int precision, bool backward, hipStream_t s,
+                         int pairs, int prefetch);
+bool loglik_kc_shape_supported(int pairs, int prefetch);
+// Wavefronts per source of the K-chunked log-likelihood kernel used for rows
+// wider than 2 KB, 0 where one wavefront per row is used.
+int loglik_kc_waves(int32_t K, int precision);
 // total = sum_s per_source[s] (fixed order)
 hipError_t launch_sum(const double* x, int64_t n, double* out, hipStream_t s);
 // grad_theta[k][0..1] = gout * dbeta_k * (-theta_k1, theta_k0) / (theta_k0+theta_k1)^2,

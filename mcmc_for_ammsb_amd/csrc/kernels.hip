@@ -1587,6 +1587,203 @@ __global__ __launch_bounds__(WAVES * 64) void k_loglik(LogLikArgs a) {
   }
 }
 
+// K-chunked form for rows too wide for one wavefront's registers (K*F above
+// 2 KB): a block of W = blockDim/64 wavefronts works on one source at a
+// time, wavefront w owning the vectors [w*CHC*64, (w+1)*CHC*64) of every row
+// (CHC coalesced 1 KB slices).  Pairs go P at a time: each wavefront loads
+// its slice of the P neighbour rows, reduces its partial Z of each pair, and
+// the W partials meet in LDS (double-buffered, one barrier per P pairs).
+// Every wavefront then holds the full Z of the P pairs and finishes its own
+// slice of dL/dphi and dL/dbeta with no further exchange; grad_phi and the
+// dbeta partials are written slice by slice.
+constexpr int kLlMaxWaves = 8;
+
+// P pairs' slices of neighbour rows, their labels and validity.
+template <typename F, int EPL, int P>
+struct LlGroup {
+  F rb[P][EPL];
+  int y[P];
+  bool ok[P];
+};
+
+// Issue the loads of pairs [p, p+P) (block-uniform ids).  A pair that is
+// absent (past p1), out of range (reported) or the source itself reads row 0
+// and is marked not ok.
+template <typename F, int CHC, int P, int EPL>
+__device__ __forceinline__ void ll_load(const LogLikArgs& a, const F* pi, int32_t node, int64_t p,
+                                        int64_t p1, int w, int lane, LlGroup<F, EPL, P>& gr) {
+  constexpr int V = VT<F>::V;
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    const bool in = p + i < p1;
+    int32_t b = in ? a.pair_nbr[p + i] : node;
+    int y = in ? (a.pair_y[p + i] ? 1 : 0) : 0;
+    b = __builtin_amdgcn_readfirstlane(b);
+    gr.y[i] = __builtin_amdgcn_readfirstlane(y);
+    const bool bad = b < 0 || b >= a.N;
+    if (bad && w == 0 && lane == 0) atomicOr(a.error, ERR_BAD_INPUT);
+    gr.ok[i] = !bad && b != node;  // mcmc_sampler_stochastic.cc:275-277
+    const F* row = pi + (int64_t)(gr.ok[i] ? b : 0) * a.ld;
+#pragma unroll
+    for (int c = 0; c < CHC; ++c) {
+      const int k0 = ((w * CHC + c) * 64 + lane) * V;
+      load_vec<F>(row + (k0 < a.K ? k0 : 0), gr.rb[i] + c * V);
+    }
+  }
+}
+
+// Partial Z of each pair over this wavefront's slice, exchange through LDS
+// (one barrier), then the forward log-sum or this slice's adjoint terms.
+template <typename F, int CHC, int P, int EPL, bool BWD>
+__device__ __forceinline__ void ll_process(LlGroup<F, EPL, P>& gr, const F* pis, const F* f1, F e1,
+                                           F e0, int K, int w, int lane, int W,
+                                           F (*s_z)[P][kLlMaxWaves], int& buf, double& ll,
+                                           F* acc, double* dbeta) {
+  constexpr int V = VT<F>::V;
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+#pragma unroll
+    for (int c = 0; c < CHC; ++c)
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const int k = ((w * CHC + c) * 64 + lane) * V + v;
+        gr.rb[i][c * V + v] = k < K ? gr.rb[i][c * V + v] : (F)0;
+      }
+    F z = (F)0;
+#pragma unroll
+    for (int j = 0; j < EPL; ++j)
+      z = fma(pis[j], gr.y[i] ? fma(gr.rb[i][j], f1[j], e1) : fma(-gr.rb[i][j], f1[j], e0), z);
+    z = group_sum<64>(z);
+    if (lane == 0) s_z[buf][i][w] = z;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    if (!gr.ok[i]) continue;
+    F Z = (F)0;
+    for (int q = 0; q < W; ++q) Z += s_z[buf][i][q];
+    if (!BWD) {
+      if (w == 0) ll += log((double)Z);
+    } else {
+      const F rz = recip(Z);
+      const double sg = gr.y[i] ? 1.0 : -1.0;
+#pragma unroll
+      for (int j = 0; j < EPL; ++j) {
+        const F x = gr.y[i] ? fma(gr.rb[i][j], f1[j], e1) : fma(-gr.rb[i][j], f1[j], e0);
+        acc[j] += fma(x, rz, (F)-1);
+        dbeta[j] += sg * (double)(pis[j] * gr.rb[i][j] * rz);
+      }
+    }
+  }
+  buf ^= 1;
+}
+
+// PF: the next P pairs' loads are issued before this group's barrier, so
+// they are in flight while the block waits and finishes the current group.
+template <typename F, int CHC, int P, bool PF, bool BWD>
+__global__ __launch_bounds__(kLlMaxWaves * 64) void k_loglik_kc(LogLikArgs a) {
+  constexpr int V = VT<F>::V;
+  constexpr int EPL = V * CHC;
+  __shared__ F s_z[2][P][kLlMaxWaves];
+  __shared__ double s_sp[kLlMaxWaves];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int W = blockDim.x >> 6;
+  const int K = a.K;
+  const F* pi = (const F*)a.pi;
+  const F eps = (F)a.epsilon;
+  const F e1 = eps, e0 = (F)1 - eps;
+  F f1[EPL];
+  double dbeta[EPL];
+#pragma unroll
+  for (int c = 0; c < CHC; ++c)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int k = ((w * CHC + c) * 64 + lane) * V + v;
+      double bk = 0.0;
+      if (k < K) bk = a.theta[2 * k + 1] / (a.theta[2 * k] + a.theta[2 * k + 1]);
+      f1[c * V + v] = (F)bk - eps;
+      dbeta[c * V + v] = 0.0;
+    }
+  int buf = 0;
+  for (int64_t s = blockIdx.x; s < a.M; s += gridDim.x) {
+    const int32_t node = a.nodes[s];
+    const F* phis = (const F*)a.phi + s * a.phi_ld;
+    F pis[EPL];
+    double sp = 0.0;
+#pragma unroll
+    for (int c = 0; c < CHC; ++c)
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const int k = ((w * CHC + c) * 64 + lane) * V + v;
+        pis[c * V + v] = k < K ? phis[k] : (F)0;
+        sp += (double)pis[c * V + v];
+      }
+    sp = group_sum<64>(sp);
+    if (lane == 0) s_sp[w] = sp;
+    __syncthreads();
+    sp = 0.0;
+    for (int q = 0; q < W; ++q) sp += s_sp[q];  // sum(phi_s), fixed order
+#pragma unroll
+    for (int j = 0; j < EPL; ++j) pis[j] = (F)((double)pis[j] / sp);
+    F acc[EPL];
+#pragma unroll
+    for (int j = 0; j < EPL; ++j) acc[j] = (F)0;
+    double ll = 0.0;
+    const int64_t p0 = min(max(a.pair_ptr[s], (int64_t)0), a.P);
+    const int64_t p1 = min(max(a.pair_ptr[s + 1], p0), a.P);
+    if (PF) {
+      LlGroup<F, EPL, P> A, B;
+      int64_t p = p0;
+      if (p < p1) ll_load<F, CHC, P, EPL>(a, pi, node, p, p1, w, lane, A);
+      while (p < p1) {
+        const bool more = p + P < p1;
+        if (more) ll_load<F, CHC, P, EPL>(a, pi, node, p + P, p1, w, lane, B);
+        ll_process<F, CHC, P, EPL, BWD>(A, pis, f1, e1, e0, K, w, lane, W, s_z, buf, ll, acc,
+                                        dbeta);
+        p += P;
+        if (!more) break;
+        const bool more2 = p + P < p1;
+        if (more2) ll_load<F, CHC, P, EPL>(a, pi, node, p + P, p1, w, lane, A);
+        ll_process<F, CHC, P, EPL, BWD>(B, pis, f1, e1, e0, K, w, lane, W, s_z, buf, ll, acc,
+                                        dbeta);
+        p += P;
+        if (!more2) break;
+      }
+    } else {
+      for (int64_t p = p0; p < p1; p += P) {
+        LlGroup<F, EPL, P> A;
+        ll_load<F, CHC, P, EPL>(a, pi, node, p, p1, w, lane, A);
+        ll_process<F, CHC, P, EPL, BWD>(A, pis, f1, e1, e0, K, w, lane, W, s_z, buf, ll, acc,
+                                        dbeta);
+      }
+    }
+    if (!BWD) {
+      if (w == 0 && lane == 0) a.per_source[s] = ll;
+    } else {
+      const double go = a.gout[0];
+      F* gp = (F*)a.grad_phi + s * a.phi_ld;
+#pragma unroll
+      for (int c = 0; c < CHC; ++c)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const int k = ((w * CHC + c) * 64 + lane) * V + v;
+          if (k < K) gp[k] = (F)(go * (double)acc[c * V + v] / sp);
+        }
+    }
+    __syncthreads();  // s_sp and s_z are rewritten by the next source
+  }
+  if (BWD) {
+    double* out = a.dbeta_partials + (int64_t)blockIdx.x * K;
+#pragma unroll
+    for (int c = 0; c < CHC; ++c)
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const int k = ((w * CHC + c) * 64 + lane) * V + v;
+        if (k < K) out[k] = dbeta[c * V + v];
+      }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_sum(const double* x, int64_t n, double* out) {
   __shared__ double s[256];
   double v = 0.0;
@@ -1600,13 +1797,22 @@ __global__ __launch_bounds__(256) void k_sum(const double* x, int64_t n, double*
   if (threadIdx.x == 0) out[0] = s[0];
 }
 
-__global__ __launch_bounds__(256) void k_theta_grad(const double* part, int blocks, int K,
-                                                    const double* theta, const double* gout,
-                                                    double* grad) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= K) return;
+// dL/dtheta from the [blocks][K] dbeta partials: 64 k per block, 16 lane
+// rows splitting the partials (b = r mod 16), fixed-order LDS combine.
+__global__ __launch_bounds__(1024) void k_theta_grad(const double* part, int blocks, int K,
+                                                     const double* theta, const double* gout,
+                                                     double* grad) {
+  __shared__ double s[16][64];
+  const int kl = threadIdx.x & 63, r = threadIdx.x >> 6;
+  const int k = blockIdx.x * 64 + kl;
   double d = 0.0;
-  for (int b = 0; b < blocks; ++b) d += part[(int64_t)b * K + k];
+  if (k < K)
+    for (int b = r; b < blocks; b += 16) d += part[(int64_t)b * K + k];
+  s[r][kl] = d;
+  __syncthreads();
+  if (r != 0 || k >= K) return;
+  d = 0.0;
+  for (int q = 0; q < 16; ++q) d += s[q][kl];
   const double t0 = theta[2 * k], t1 = theta[2 * k + 1];
   const double S = t0 + t1;
   const double go = gout[0];
@@ -1616,12 +1822,52 @@ __global__ __launch_bounds__(256) void k_theta_grad(const double* part, int bloc
 
 }  // namespace
 
-hipError_t launch_loglik(const LogLikArgs& a, int precision, bool backward, hipStream_t s) {
+int loglik_kc_waves(int32_t K, int precision) {
+  const int V = precision == 0 ? 2 : 4;
+  const int vecs = (K + V - 1) / V;
+  if (vecs <= 2 * 64) return 0;  // one wavefront holds the row
+  const int w = (vecs + 2 * 64 - 1) / (2 * 64);
+  return w <= kLlMaxWaves ? w : 0;
+}
+
+bool loglik_kc_shape_supported(int pairs, int prefetch) {
+  return (pairs == 2 || pairs == 4 || pairs == 8) && (prefetch == 0 || prefetch == 1);
+}
+
+hipError_t launch_loglik(const LogLikArgs& a, int precision, bool backward, hipStream_t s,
+                         int pairs, int prefetch) {
   int lpr, ch;
   if (!row_config(a.K, precision, &lpr, &ch)) return hipErrorInvalidValue;
   if (a.M <= 0) return hipSuccess;
+  const int waves = loglik_kc_waves(a.K, precision);
+  if (waves > 0) {
+    const dim3 grid(a.blocks), block(waves * 64);
+#define KC(P, PF)                                                                               \
+  if (pairs == P && prefetch == PF) {                                                           \
+    if (precision == 0) {                                                                       \
+      if (backward)                                                                             \
+        hipLaunchKernelGGL((k_loglik_kc<double, 2, P, PF, true>), grid, block, 0, s, a);        \
+      else                                                                                      \
+        hipLaunchKernelGGL((k_loglik_kc<double, 2, P, PF, false>), grid, block, 0, s, a);       \
+    } else {                                                                                    \
+      if (backward)                                                                             \
+        hipLaunchKernelGGL((k_loglik_kc<float, 2, P, PF, true>), grid, block, 0, s, a);         \
+      else                                                                                      \
+        hipLaunchKernelGGL((k_loglik_kc<float, 2, P, PF, false>), grid, block, 0, s, a);        \
+    }                                                                                           \
+    return hipGetLastError();                                                                   \
+  }
+    KC(2, false)
+    KC(2, true)
+    KC(4, false)
+    KC(4, true)
+    KC(8, false)
+    KC(8, true)
+#undef KC
+    return hipErrorInvalidValue;
+  }
   const dim3 grid(a.blocks), block(4 * 64);
-#define X(L, C)                                                                                 \
+#define X(L, C)                                                                              \
   if (lpr == L && ch == C) {                                                                    \
     if (precision == 0) {                                                                       \
       if (backward)                                                                             \
@@ -1648,7 +1894,7 @@ hipError_t launch_sum(const double* x, int64_t n, double* out, hipStream_t s) {
 
 hipError_t launch_theta_grad(const double* part, int32_t blocks, int32_t K, const double* theta,
                              const double* gout, double* grad, hipStream_t s) {
-  hipLaunchKernelGGL(k_theta_grad, dim3((K + 255) / 256), dim3(256), 0, s, part, blocks, K, theta,
+  hipLaunchKernelGGL(k_theta_grad, dim3((K + 63) / 64), dim3(1024), 0, s, part, blocks, K, theta,
                      gout, grad);
   return hipGetLastError();
 }

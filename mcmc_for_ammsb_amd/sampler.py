@@ -18,6 +18,11 @@ PHASES = ("outer", "perplexity", "sample_mini_batch", "nodes_in_mini_batch",
           "sample_neighbor_nodes", "update_phi", "update_pi", "update_beta")
 
 
+# set in this order: each launch-shape pair is validated on its second key
+TUNING_KEYS = ("phi_waves", "phi_unroll", "ll_fwd_pairs", "ll_fwd_prefetch", "ll_bwd_pairs",
+               "ll_bwd_prefetch")
+
+
 class MCMCSamplerStochastic:
     """SGRLD sampler for the assortative MMSB with HBM-resident state.
 
@@ -184,8 +189,9 @@ class MCMCSamplerStochastic:
         return {p: (float(s), int(t)) for p, s, t in zip(PHASES, secs, ticks)}
 
     def set_tuning(self, **kw):
-        """Launch shape of update_phi, e.g. set_tuning(phi_waves=8, phi_unroll=2)."""
-        for name in ("phi_waves", "phi_unroll"):
+        """Launch shapes, e.g. set_tuning(phi_waves=8, phi_unroll=2) for update_phi
+        or set_tuning(ll_bwd_pairs=2, ll_bwd_prefetch=1) for the log-likelihood adjoint."""
+        for name in TUNING_KEYS:
             if name in kw:
                 _lib.check(_lib.lib.ammsb_set_tuning(self._h, name.encode(), int(kw.pop(name))))
         if kw:
@@ -193,7 +199,7 @@ class MCMCSamplerStochastic:
 
     def get_tuning(self):
         out = {}
-        for name in ("phi_waves", "phi_unroll"):
+        for name in TUNING_KEYS:
             v = ctypes.c_int64()
             _lib.check(_lib.lib.ammsb_get_tuning(self._h, name.encode(), ctypes.byref(v)))
             out[name] = v.value

@@ -9,6 +9,9 @@ Pins:
   * dL/dtheta: the reference's update_beta gradient (oracle) to 1e-12;
     central finite differences on every coordinate
   * FP32 precision mode against FP64 to 1e-4
+K = 48 runs the one-wavefront-per-row kernel; K = 600 (ragged last chunk)
+and K = 2048 run the K-chunked kernel (several wavefronts per row, Z
+exchanged through LDS).
 """
 import numpy as np
 import pytest
@@ -21,11 +24,13 @@ import mcmc_for_ammsb_amd.autograd  # noqa: E402,F401  (M.autograd)
 
 pytestmark = pytest.mark.gpu
 
-N, K, MSRC, NNL = 20000, 48, 40, 24
+MSRC, NNL = 40, 24
+SHAPES = [(20000, 48), (8000, 600), (4000, 2048)]
 
 
-@pytest.fixture(scope="module")
-def setup():
+@pytest.fixture(scope="module", params=SHAPES, ids=[f"K{k}" for _, k in SHAPES])
+def setup(request):
+    N, K = request.param
     g = M.generate("community", N=N, K=K, mean_degree=20.0, seed=9)
     s = M.make_split(g, 0.1, 42)
     rng = np.random.default_rng(3)
@@ -43,12 +48,14 @@ def _dev(x, dt=None):
 
 
 def _sampler(g, s, pi, theta, precision="fp64"):
+    K = theta.shape[0]
     smp = M.MCMCSamplerStochastic(M.Options(K=K, num_node_sample=10, precision=precision), g, s)
     smp.set_state(pi.astype(np.float64 if precision == "fp64" else np.float32), theta)
     return smp
 
 
 def _inputs(pi, theta, nodes, ptr, nbr, y, dt=torch.float64):
+    K = theta.shape[0]
     phi_src = pi[nodes, :K] * pi[nodes, K:K + 1]
     return (_dev(phi_src, dt).requires_grad_(True), _dev(theta).requires_grad_(True), _dev(nodes),
             _dev(ptr), _dev(nbr), _dev(y))
@@ -56,12 +63,13 @@ def _inputs(pi, theta, nodes, ptr, nbr, y, dt=torch.float64):
 
 def test_forward_matches_brute_force(setup):
     g, s, pi, theta, nodes, ptr, nbr, y = setup
+    N, K = g.N, theta.shape[0]
     smp = _sampler(g, s, pi, theta)
     phi_t, th_t, n_t, p_t, b_t, y_t = _inputs(pi, theta, nodes, ptr, nbr, y)
     per = M.autograd.loglik_per_source(smp, phi_t, th_t, n_t, p_t, b_t, y_t).cpu().numpy()
     beta = theta[:, 1] / theta.sum(1)
     eps = 1e-7
-    for q in range(4):
+    for q in range(4 if K <= 600 else 2):
         ref = 0.0
         pa = pi[nodes[q], :K]
         for p in range(ptr[q], ptr[q + 1]):
@@ -76,6 +84,7 @@ def test_forward_matches_brute_force(setup):
 
 def test_adjoint_matches_reference_closed_form(setup):
     g, s, pi, theta, nodes, ptr, nbr, y = setup
+    N, K = g.N, theta.shape[0]
     smp = _sampler(g, s, pi, theta)
     orc = Oracle(g, s, K, num_node_sample=10)
     orc.set_state(pi, theta)
@@ -95,6 +104,7 @@ def test_adjoint_matches_reference_closed_form(setup):
 
 def test_adjoint_matches_finite_differences(setup):
     g, s, pi, theta, nodes, ptr, nbr, y = setup
+    N, K = g.N, theta.shape[0]
     smp = _sampler(g, s, pi, theta)
     phi_t, th_t, n_t, p_t, b_t, y_t = _inputs(pi, theta, nodes, ptr, nbr, y)
     L = M.autograd.loglik(smp, phi_t, th_t, n_t, p_t, b_t, y_t)
@@ -115,7 +125,8 @@ def test_adjoint_matches_finite_differences(setup):
         fd = (vals[0] - vals[1]) / (2 * h)
         assert fd == pytest.approx(gphi[q, k], rel=1e-5, abs=1e-6), (q, k)
     base_th = th_t.detach().clone()
-    for k in range(K):
+    ks = range(K) if K <= 64 else rng.choice(K, size=48, replace=False)
+    for k in ks:
         for i in range(2):
             vals = []
             for sgn in (1, -1):
@@ -128,6 +139,7 @@ def test_adjoint_matches_finite_differences(setup):
 
 def test_fp32_against_fp64(setup):
     g, s, pi, theta, nodes, ptr, nbr, y = setup
+    N, K = g.N, theta.shape[0]
     out = {}
     for prec, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
         smp = _sampler(g, s, pi, theta, prec)
@@ -142,9 +154,35 @@ def test_fp32_against_fp64(setup):
 
 def test_bad_neighbour_id_is_an_error(setup):
     g, s, pi, theta, nodes, ptr, nbr, y = setup
+    N, K = g.N, theta.shape[0]
     smp = _sampler(g, s, pi, theta)
     bad = nbr.copy()
     bad[3] = N + 7
     phi_t, th_t, n_t, p_t, b_t, y_t = _inputs(pi, theta, nodes, ptr, bad, y)
     with pytest.raises(M.AmmsbError):
         M.autograd.loglik(smp, phi_t, th_t, n_t, p_t, b_t, y_t)
+
+
+def test_kchunk_launch_shapes_agree(setup):
+    """Every (ll_pairs, ll_prefetch) shape of the K-chunked kernels sums in
+    the same order, so values and gradients are bit-identical."""
+    g, s, pi, theta, nodes, ptr, nbr, y = setup
+    if theta.shape[0] < 600:
+        pytest.skip("one wavefront per row at this K")
+    for prec, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        smp = _sampler(g, s, pi, theta, prec)
+        phi_t, th_t, n_t, p_t, b_t, y_t = _inputs(pi, theta, nodes, ptr, nbr, y, dt)
+        ref = None
+        for pairs, pf in ((4, 1), (2, 0), (2, 1), (4, 0), (8, 0), (8, 1)):
+            smp.set_tuning(ll_fwd_pairs=pairs, ll_fwd_prefetch=pf, ll_bwd_pairs=pairs,
+                           ll_bwd_prefetch=pf)
+            per = M.autograd.loglik_per_source(smp, phi_t.detach(), th_t.detach(), n_t, p_t, b_t, y_t)
+            gp, gt = M.autograd.loglik_grad(smp, phi_t.detach(), th_t.detach(), n_t, p_t, b_t, y_t)
+            got = (per.cpu().numpy(), gp.cpu().numpy(), gt.cpu().numpy())
+            if ref is None:
+                ref = got
+            else:
+                for r, o in zip(ref, got):
+                    np.testing.assert_array_equal(o, r, err_msg=f"{prec} shape ({pairs},{pf})")
+        with pytest.raises(M.AmmsbError):
+            smp.set_tuning(ll_bwd_pairs=3)

@@ -118,3 +118,28 @@ def test_graph_from_edges_and_file(tmp_path):
 def test_sampler_requires_graph_type():
     with pytest.raises(TypeError):
         M.MCMCSamplerStochastic(Options(K=4), "not a graph")
+
+
+def test_make_pairs_invariants():
+    """Pairs of the differentiation workload: every training edge of each
+    source exactly once (held-out/test excluded), then distinct non-edges
+    that avoid the source, its neighbours and the held-out/test pairs."""
+    pytest.importorskip("torch")
+    from mcmc_for_ammsb_amd.autograd import make_pairs
+    g = M.generate("community", N=20000, K=48, mean_degree=20.0, seed=9)
+    s = M.make_split(g, 0.1, 42)
+    nodes = np.sort(np.random.default_rng(3).choice(g.N, 40, replace=False))
+    ptr, nbr, y = make_pairs(g, s, nodes, 24, seed=1)
+    assert ptr[0] == 0 and ptr[-1] == len(nbr) == len(y)
+    ex = np.sort(np.concatenate([s.held_out_edges, s.test_edges]), 1)
+    ht = set(map(tuple, ex.tolist()))
+    for q, sn in enumerate(nodes):
+        row = set(g.cols[g.row_ptr[sn]:g.row_ptr[sn + 1]].tolist())
+        nb, yy = nbr[ptr[q]:ptr[q + 1]], y[ptr[q]:ptr[q + 1]]
+        assert len(set(nb.tolist())) == len(nb)
+        assert (yy == 0).sum() == 24
+        for v, lab in zip(nb.tolist(), yy.tolist()):
+            assert v != sn and (min(sn, v), max(sn, v)) not in ht
+            assert (v in row) == bool(lab)
+        train = {v for v in row if (min(sn, v), max(sn, v)) not in ht}
+        assert train == set(nb[yy == 1].tolist())
