@@ -327,6 +327,60 @@ int ammsb_loglik_bwd(ammsb_ctx *ctx, int64_t M, const int32_t *nodes, const void
  * beta ([K] double). */
 int ammsb_device_ptrs(ammsb_ctx *ctx, void **pi, void **theta, void **beta);
 
+/* ------------------------------------------------------------------------
+ * Batches of independent chains (no counterpart in the reference, which
+ * runs one sampler per process; each chain is the sampler of
+ * mcmc_sampler_stochastic.cc:93-183 with its own graph, split, options,
+ * K and seed).  All chains of a batch share one precision, one device and
+ * one stream.  Every phase of an iteration is one launch over all chains
+ * (update_phi a persistent kernel fed by a work queue), and the launch
+ * sequence of an iteration is captured once as a hipGraph and replayed.
+ * Limits: 1..1024 chains; K <= 256 (FP64) / 512 (FP32); stratified
+ * minibatch strategies.  Chains draw their minibatch and neighbours every
+ * iteration from the same Philox streams as a single context with the
+ * chain's options and seed, so a chain's draws equal those of ammsb_step
+ * on such a context. */
+typedef struct ammsb_batch ammsb_batch;
+
+/* opts, graphs: [n]; splits: [n] or NULL (no held-out set).  The arrays
+ * are copied. */
+int ammsb_batch_create(int32_t n, const ammsb_options *opts, const ammsb_graph *graphs,
+                       const ammsb_split *splits, ammsb_batch **out);
+int ammsb_batch_destroy(ammsb_batch *b);
+/* Chain i as a context owned by the batch, for the single-context calls
+ * (ammsb_get_info, ammsb_init_state, ammsb_set_state, ammsb_get_state,
+ * ammsb_get_rows, ammsb_perplexity, ...).  ammsb_destroy refuses it. */
+int ammsb_batch_chain(ammsb_batch *b, int32_t i, ammsb_ctx **chain);
+/* init_state of every chain with seeds [n] (NULL: each chain's
+ * options.random_seed). */
+int ammsb_batch_init_state(ammsb_batch *b, const uint64_t *seeds);
+/* `iters` iterations of every chain (graph replays unless disabled). */
+int ammsb_batch_step(ammsb_batch *b, int64_t iters);
+/* Held-out perplexity of every chain (cal_perplexity_held_out,
+ * learner.cc:126-183) into out [n]. */
+int ammsb_batch_perplexity(ammsb_batch *b, double *out);
+/* 1: replay the captured hipGraph (default); 0: launch the kernels. */
+int ammsb_batch_use_graph(ammsb_batch *b, int32_t enable);
+
+/* Launch shape of chain i in the batched update_phi:
+ *   lanes_per_row (4..64) x vectors_per_lane (1, 2) of 16 bytes = the
+ *   K-chunk each lane owns; nodes_per_block = minibatch nodes a workgroup
+ *   of 4 wavefronts processes together (power of two; nodes per wavefront
+ *   = nodes_per_block / 4).  The shape changes summation order only. */
+typedef struct ammsb_batch_shape {
+  int32_t lanes_per_row;
+  int32_t vectors_per_lane;
+  int32_t nodes_per_block;
+  double us;   /* autotune: measured update_phi time of the chain, else 0 */
+} ammsb_batch_shape;
+int ammsb_batch_set_shape(ammsb_batch *b, int32_t i, const ammsb_batch_shape *shape);
+int ammsb_batch_get_shape(ammsb_batch *b, int32_t i, ammsb_batch_shape *shape);
+/* Autotuner: for every chain, times each valid shape on the chain's
+ * current minibatch (reps launches each, HIP events) and keeps the
+ * fastest; chosen shapes into out [n] (may be NULL).  The state is not
+ * changed.  Requires initialised state. */
+int ammsb_batch_autotune(ammsb_batch *b, int32_t reps, ammsb_batch_shape *out);
+
 #ifdef __cplusplus
 }
 #endif

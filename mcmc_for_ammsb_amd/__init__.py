@@ -8,7 +8,7 @@ HIP kernels for gfx950 behind the C-ABI of ``libammsb.so``
 from ._lib import AmmsbError, FP32, FP64, lib  # noqa: F401  (fails loudly if unbuilt)
 from .graph import Graph, Split, generate, make_split, philox  # noqa: F401
 from .options import Options  # noqa: F401
-from .sampler import MCMCSamplerStochastic  # noqa: F401
+from .sampler import MCMCBatch, MCMCSamplerStochastic  # noqa: F401
 
 __all__ = ["AmmsbError", "Graph", "Split", "generate", "make_split", "philox", "Options",
-           "MCMCSamplerStochastic"]
+           "MCMCSamplerStochastic", "MCMCBatch"]

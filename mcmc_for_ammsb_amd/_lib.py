@@ -84,6 +84,11 @@ class GenParams(ctypes.Structure):
     ]
 
 
+class BatchShape(ctypes.Structure):
+    _fields_ = [("lanes_per_row", c_i32), ("vectors_per_lane", c_i32),
+                ("nodes_per_block", c_i32), ("us", c_dbl)]
+
+
 class AmmsbError(RuntimeError):
     """Raised for every non-zero status of the C-ABI."""
 
@@ -167,6 +172,17 @@ def _load():
         "ammsb_get_tuning": (c_i32, [c_vp, ctypes.c_char_p, P_i64]),
         "ammsb_device_ptrs": (c_i32, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp),
                                       ctypes.POINTER(c_vp)]),
+        "ammsb_batch_create": (c_i32, [c_i32, ctypes.POINTER(Options), ctypes.POINTER(Graph),
+                                       ctypes.POINTER(Split), ctypes.POINTER(c_vp)]),
+        "ammsb_batch_destroy": (c_i32, [c_vp]),
+        "ammsb_batch_chain": (c_i32, [c_vp, c_i32, ctypes.POINTER(c_vp)]),
+        "ammsb_batch_init_state": (c_i32, [c_vp, ctypes.POINTER(c_u64)]),
+        "ammsb_batch_step": (c_i32, [c_vp, c_i64]),
+        "ammsb_batch_perplexity": (c_i32, [c_vp, P_dbl]),
+        "ammsb_batch_use_graph": (c_i32, [c_vp, c_i32]),
+        "ammsb_batch_set_shape": (c_i32, [c_vp, c_i32, ctypes.POINTER(BatchShape)]),
+        "ammsb_batch_get_shape": (c_i32, [c_vp, c_i32, ctypes.POINTER(BatchShape)]),
+        "ammsb_batch_autotune": (c_i32, [c_vp, c_i32, ctypes.POINTER(BatchShape)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

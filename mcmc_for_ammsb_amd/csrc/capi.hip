@@ -67,6 +67,9 @@ struct ammsb_ctx {
   int64_t edge_cap = 0, node_cap = 0;
   int64_t max_degree = 0;
   hipStream_t stream = nullptr;
+  bool own_stream = true;           // false: the stream of the batch that owns the chain
+  ammsb_batch* owner = nullptr;     // batch that owns this chain, if any
+  int32_t max_ring_slots = 16;
   Key key{};
   DevHyper hp{};
   // device graph
@@ -214,38 +217,119 @@ struct PhaseScope {
   }
 };
 
+// ---- kernel arguments of one iteration, built from the context
+SamplerArgs sampler_args(const ammsb_ctx* c, int64_t t0) {
+  SamplerArgs sa;
+  sa.g = dev_graph(c);
+  sa.st = c->d_st;
+  sa.key = c->key;
+  sa.strategy = c->opt.strategy;
+  sa.m = c->m;
+  sa.max_source = c->opt.sampler_max_source;
+  sa.ring = c->ring;
+  sa.t0 = t0;
+  sa.table = c->d_mbtable;
+  sa.table_cap = c->mbtable_cap;
+  sa.table_in_lds = c->mbtable_in_lds;
+  return sa;
+}
+
+NeighborArgs neighbor_args(const ammsb_ctx* c) {
+  NeighborArgs na;
+  na.g = dev_graph(c);
+  na.st = c->d_st;
+  na.key = c->key;
+  na.ring = c->ring;
+  na.gtable = c->d_nbr_gtable;
+  na.tcap = c->nbr_tcap;
+  na.use_lds = c->nbr_use_lds;
+  na.gtable_slots = c->nbr_gslots;
+  return na;
+}
+
+PhiArgs phi_args(const ammsb_ctx* c) {
+  PhiArgs pa;
+  pa.pi = c->d_pi;
+  pa.ld = c->ld;
+  pa.K = c->K;
+  pa.N = c->N;
+  pa.use_ring = 1;
+  pa.ring = c->ring;
+  pa.nodes = nullptr;
+  pa.count_host = 0;
+  pa.nbrs = nullptr;
+  pa.nnb = (int32_t)c->nnb;
+  pa.beta = c->d_beta;
+  pa.st = c->d_st;
+  pa.hp = c->hp;
+  pa.eps_override = -1.0;
+  pa.noise = nullptr;
+  pa.key = c->key;
+  pa.staging = c->d_staging;
+  return pa;
+}
+
+ScatterArgs scatter_args(const ammsb_ctx* c) {
+  ScatterArgs sc;
+  sc.pi = c->d_pi;
+  sc.ld = c->ld;
+  sc.K = c->K;
+  sc.use_ring = 1;
+  sc.ring = c->ring;
+  sc.st = c->d_st;
+  sc.nodes = nullptr;
+  sc.count_host = 0;
+  sc.staging = c->d_staging;
+  return sc;
+}
+
+BetaArgs beta_args(const ammsb_ctx* c) {
+  BetaArgs ba;
+  ba.pi = c->d_pi;
+  ba.ld = c->ld;
+  ba.K = c->K;
+  ba.use_ring = 1;
+  ba.ring = c->ring;
+  ba.st = c->d_st;
+  ba.edges = nullptr;
+  ba.y = nullptr;
+  ba.count_host = 0;
+  ba.beta = c->d_beta;
+  ba.theta = c->d_theta;
+  ba.epsilon = c->hp.epsilon;
+  ba.partials = c->d_beta_partials;
+  ba.blocks = c->beta_blocks;
+  return ba;
+}
+
+ThetaArgs theta_args(const ammsb_ctx* c) {
+  ThetaArgs ta;
+  ta.theta = c->d_theta;
+  ta.beta = c->d_beta;
+  ta.K = c->K;
+  ta.partials = c->d_beta_partials;
+  ta.blocks = c->beta_blocks;
+  ta.st = c->d_st;
+  ta.hp = c->hp;
+  ta.eps_override = -1.0;
+  ta.scale_override = -1.0;
+  ta.meta = c->ring.meta;
+  ta.noise = nullptr;
+  ta.key = c->key;
+  ta.bump_step = 1;
+  return ta;
+}
+
 // ---- sample-ahead: draw `fills` iterations starting at step t0 (t0 < 0: the
 // device's step_count) into ring slots 0..fills-1
 int fill_ring(ammsb_ctx* c, int64_t t0, int32_t fills) {
-  const DevGraph g = dev_graph(c);
   {
     PhaseScope ps(c, PH_MINIBATCH);
-    SamplerArgs sa;
-    sa.g = g;
-    sa.st = c->d_st;
-    sa.key = c->key;
-    sa.strategy = c->opt.strategy;
-    sa.m = c->m;
-    sa.max_source = c->opt.sampler_max_source;
-    sa.ring = c->ring;
-    sa.t0 = t0;
-    sa.table = c->d_mbtable;
-    sa.table_cap = c->mbtable_cap;
-    sa.table_in_lds = c->mbtable_in_lds;
-    HIP_TRY(launch_minibatch(sa, fills, c->stream));
+    HIP_TRY(launch_minibatch(sampler_args(c, t0), fills, c->stream));
   }
   {
     PhaseScope ps(c, PH_NEIGHBORS);
-    NeighborArgs na;
-    na.g = g;
-    na.st = c->d_st;
-    na.key = c->key;
-    na.ring = c->ring;
-    na.gtable = c->d_nbr_gtable;
-    na.tcap = c->nbr_tcap;
-    na.use_lds = c->nbr_use_lds;
-    na.gtable_slots = c->nbr_gslots;
-    HIP_TRY(launch_neighbors(na, fills, c->stream));
+    HIP_TRY(launch_neighbors(neighbor_args(c), fills, c->stream));
   }
   return AMMSB_OK;
 }
@@ -262,74 +346,17 @@ int enqueue_iteration(ammsb_ctx* c) {
   }
   {
     PhaseScope ps(c, PH_UPDATE_PHI);
-    PhiArgs pa;
-    pa.pi = c->d_pi;
-    pa.ld = c->ld;
-    pa.K = c->K;
-    pa.N = c->N;
-    pa.use_ring = 1;
-    pa.ring = c->ring;
-    pa.nodes = nullptr;
-    pa.count_host = 0;
-    pa.nbrs = nullptr;
-    pa.nnb = (int32_t)c->nnb;
-    pa.beta = c->d_beta;
-    pa.st = c->d_st;
-    pa.hp = c->hp;
-    pa.eps_override = -1.0;
-    pa.noise = nullptr;
-    pa.key = c->key;
-    pa.staging = c->d_staging;
-    HIP_TRY(launch_update_phi(pa, c->prec, (int32_t)c->node_cap, c->stream, c->phi_waves,
+    HIP_TRY(launch_update_phi(phi_args(c), c->prec, (int32_t)c->node_cap, c->stream, c->phi_waves,
                               c->phi_unroll));
   }
   {
     PhaseScope ps(c, PH_UPDATE_PI);
-    ScatterArgs sc;
-    sc.pi = c->d_pi;
-    sc.ld = c->ld;
-    sc.K = c->K;
-    sc.use_ring = 1;
-    sc.ring = c->ring;
-    sc.st = c->d_st;
-    sc.nodes = nullptr;
-    sc.count_host = 0;
-    sc.staging = c->d_staging;
-    HIP_TRY(launch_scatter(sc, c->prec, (int32_t)c->node_cap, c->stream));
+    HIP_TRY(launch_scatter(scatter_args(c), c->prec, (int32_t)c->node_cap, c->stream));
   }
   {
     PhaseScope ps(c, PH_UPDATE_BETA);
-    BetaArgs ba;
-    ba.pi = c->d_pi;
-    ba.ld = c->ld;
-    ba.K = c->K;
-    ba.use_ring = 1;
-    ba.ring = c->ring;
-    ba.st = c->d_st;
-    ba.edges = nullptr;
-    ba.y = nullptr;
-    ba.count_host = 0;
-    ba.beta = c->d_beta;
-    ba.theta = c->d_theta;
-    ba.epsilon = c->hp.epsilon;
-    ba.partials = c->d_beta_partials;
-    ba.blocks = c->beta_blocks;
-    HIP_TRY(launch_beta(ba, c->prec, c->stream));
-    ThetaArgs ta;
-    ta.theta = c->d_theta;
-    ta.beta = c->d_beta;
-    ta.K = c->K;
-    ta.partials = c->d_beta_partials;
-    ta.blocks = c->beta_blocks;
-    ta.st = c->d_st;
-    ta.hp = c->hp;
-    ta.eps_override = -1.0;
-    ta.scale_override = -1.0;
-    ta.meta = c->ring.meta;
-    ta.noise = nullptr;
-    ta.key = c->key;
-    ta.bump_step = 1;
-    HIP_TRY(launch_theta(ta, c->stream));
+    HIP_TRY(launch_beta(beta_args(c), c->prec, c->stream));
+    HIP_TRY(launch_theta(theta_args(c), c->stream));
   }
   return AMMSB_OK;
 }
@@ -426,7 +453,7 @@ void free_ctx(ammsb_ctx* c) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -509,7 +536,7 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
   if (o.precision == AMMSB_FP32) c->hp.Nn = (double)(float)((1.0f * (float)c->N) / (float)c->n);
 
   HIP_TRY(hipSetDevice(o.device));
-  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  if (!c->stream) HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
   // ---- graph
   for (int32_t i = 0; i < c->N; ++i)
@@ -594,7 +621,8 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
     // sample-ahead ring: up to 16 iterations, at most ~1 GiB
     const int64_t slot_bytes = c->edge_cap * 9 + c->node_cap * 4 + c->node_cap * c->nnb * 4 +
                                (int64_t)c->N * 4 + 64;
-    const int64_t slots = std::max<int64_t>(1, std::min<int64_t>(16, ((int64_t)1 << 30) / slot_bytes));
+    const int64_t slots = std::max<int64_t>(
+        1, std::min<int64_t>(c->max_ring_slots, ((int64_t)1 << 30) / slot_bytes));
     Ring& R = c->ring;
     R.slots = (int32_t)slots;
     R.edge_cap = c->edge_cap;
@@ -736,6 +764,10 @@ int ammsb_create(const ammsb_options* opts, const ammsb_graph* g, const ammsb_sp
 
 int ammsb_destroy(ammsb_ctx* c) {
   if (!c) return AMMSB_OK;
+  if (c->owner) {
+    set_error("ammsb_destroy: the context is a chain of a batch (ammsb_batch_destroy frees it)");
+    return AMMSB_ERR_ARG;
+  }
   (void)hipSetDevice(c->opt.device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_ctx(c);
@@ -1334,6 +1366,419 @@ int ammsb_device_ptrs(ammsb_ctx* c, void** pi, void** theta, void** beta) {
   if (pi) *pi = c->d_pi;
   if (theta) *theta = c->d_theta;
   if (beta) *beta = c->d_beta;
+  return AMMSB_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batches of independent chains.  The chains are full contexts (graph,
+// split, state, ring of one slot) on the batch's stream; the batch keeps
+// device arrays of each phase's per-chain kernel arguments, so one launch
+// per phase serves every chain and the iteration's launch sequence is fixed:
+// it is captured once as a hipGraph and replayed.
+struct ammsb_batch {
+  int32_t n = 0;
+  int prec = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<ammsb_ctx*> chains;
+  SamplerArgs* d_sargs = nullptr;
+  NeighborArgs* d_nargs = nullptr;
+  PhiArgs* d_pargs = nullptr;
+  BetaArgs* d_bargs = nullptr;
+  ThetaArgs* d_targs = nullptr;
+  BatchPhiShape* d_shapes = nullptr;
+  int32_t* d_beta_cfg = nullptr;
+  int32_t* d_index = nullptr;     // items [n+1], nodes [n+1], counter
+  int32_t* d_tune_index = nullptr;  // the same for one chain (autotuner)
+  std::vector<BatchPhiShape> shapes;
+  std::vector<double> tuned_us;
+  size_t mb_lds = 0, nb_lds = 0;
+  int32_t max_node_cap = 0, max_beta_blocks = 0, max_K = 0;
+  int32_t phi_grid = 1, finish_grid = 1;
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+BatchIndex batch_index(int32_t* base, int32_t n) {
+  BatchIndex ix;
+  ix.items = base;
+  ix.nodes = base + (n + 1);
+  ix.counter = base + 2 * (n + 1);
+  return ix;
+}
+
+void batch_drop_graph(ammsb_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+void free_batch(ammsb_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  for (auto* c : b->chains) free_ctx(c);
+  void* ptrs[] = {b->d_sargs,  b->d_nargs,    b->d_pargs, b->d_bargs,      b->d_targs,
+                  b->d_shapes, b->d_beta_cfg, b->d_index, b->d_tune_index};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int batch_upload_shapes(ammsb_batch* b) {
+  HIP_TRY(hipMemcpyAsync(b->d_shapes, b->shapes.data(), sizeof(BatchPhiShape) * b->n,
+                         hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return AMMSB_OK;
+}
+
+// The samplers of the current iteration (ring slot 0 of every chain) and the
+// work index; no state changes (draws depend on the step only).
+int batch_enqueue_samplers(ammsb_batch* b) {
+  HIP_TRY(launch_minibatch_batch(b->d_sargs, b->n, b->mb_lds, b->stream));
+  HIP_TRY(launch_neighbors_batch(b->d_nargs, b->n, b->max_node_cap, b->nb_lds, b->stream));
+  HIP_TRY(launch_batch_index(b->d_pargs, b->d_shapes, b->n, batch_index(b->d_index, b->n),
+                             b->stream));
+  return AMMSB_OK;
+}
+
+int batch_enqueue_iteration(ammsb_batch* b) {
+  int rc = batch_enqueue_samplers(b);
+  if (rc != AMMSB_OK) return rc;
+  const BatchIndex ix = batch_index(b->d_index, b->n);
+  HIP_TRY(launch_update_phi_batch(b->d_pargs, b->d_shapes, b->n, ix, b->prec, b->phi_grid,
+                                  b->stream));
+  HIP_TRY(launch_phi_finish_batch(b->d_pargs, b->n, ix, b->prec, b->finish_grid, b->stream));
+  HIP_TRY(launch_beta_batch(b->d_bargs, b->d_beta_cfg, b->n, b->max_beta_blocks, b->prec,
+                            b->stream));
+  HIP_TRY(launch_theta_batch(b->d_targs, b->n, b->max_K, b->stream));
+  return AMMSB_OK;
+}
+
+int batch_capture(ammsb_batch* b) {
+  batch_drop_graph(b);
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = batch_enqueue_iteration(b);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc != AMMSB_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HIP_TRY(e);
+  b->graph = g;
+  HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return AMMSB_OK;
+}
+
+int batch_check_states(ammsb_batch* b) {
+  for (auto* c : b->chains) {
+    const int rc = check_devstate(c);
+    if (rc != AMMSB_OK) return rc;
+  }
+  return AMMSB_OK;
+}
+
+int shape_cfg(const ammsb_batch_shape& s) {
+  for (int i = 0; i < kBatchCfgs; ++i)
+    if (kBatchCfgLpr[i] == s.lanes_per_row && kBatchCfgCh[i] == s.vectors_per_lane) return i;
+  return -1;
+}
+
+bool shape_valid(const ammsb_ctx* c, int cfg, int npb) {
+  if (!batch_cfg_valid(cfg, c->K, c->prec)) return false;
+  const int groups = kBatchWaves * (64 / kBatchCfgLpr[cfg]);
+  return npb >= 1 && npb <= groups && (npb & (npb - 1)) == 0;
+}
+
+// Default nodes per workgroup: about four neighbour rows per row group.
+int default_npb(const ammsb_ctx* c, int cfg) {
+  const int groups = kBatchWaves * (64 / kBatchCfgLpr[cfg]);
+  int gpn = 1;
+  while (gpn * 2 <= groups && gpn * 2 * 4 <= c->nnb) gpn *= 2;
+  return groups / gpn;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ammsb_batch_create(int32_t n, const ammsb_options* opts, const ammsb_graph* graphs,
+                       const ammsb_split* splits, ammsb_batch** out) {
+  if (!out || !opts || !graphs || n < 1 || n > kMaxBatchChains) {
+    set_error("ammsb_batch_create: need 1..1024 chains with options and graphs");
+    return AMMSB_ERR_ARG;
+  }
+  *out = nullptr;
+  for (int32_t i = 0; i < n; ++i) {
+    if (opts[i].precision != opts[0].precision || opts[i].device != opts[0].device) {
+      set_error("ammsb_batch_create: all chains must share precision and device");
+      return AMMSB_ERR_ARG;
+    }
+    if (opts[i].strategy == AMMSB_RANDOM_EDGE) {
+      set_error("ammsb_batch_create: batches use the stratified minibatch strategies");
+      return AMMSB_ERR_ARG;
+    }
+    if (batch_cfg_default((int32_t)opts[i].K, opts[i].precision) < 0) {
+      set_error("ammsb_batch_create: K of chain " + std::to_string(i) +
+                " exceeds the batched kernels (256 FP64 / 512 FP32)");
+      return AMMSB_ERR_ARG;
+    }
+  }
+  ammsb_batch* b = nullptr;
+  try {
+    b = new ammsb_batch();
+  } catch (...) {
+    set_error("ammsb_batch_create: out of host memory");
+    return AMMSB_ERR_OOM;
+  }
+  b->n = n;
+  b->prec = opts[0].precision;
+  b->device = opts[0].device;
+  auto fail = [&](int rc) {
+    free_batch(b);
+    return rc;
+  };
+  if (hipSetDevice(b->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("ammsb_batch_create: cannot create a stream on the device");
+    return fail(AMMSB_ERR_HIP);
+  }
+  std::vector<SamplerArgs> sargs(n);
+  std::vector<NeighborArgs> nargs(n);
+  std::vector<PhiArgs> pargs(n);
+  std::vector<BetaArgs> bargs(n);
+  std::vector<ThetaArgs> targs(n);
+  std::vector<int32_t> beta_cfg(n);
+  b->shapes.resize(n);
+  b->tuned_us.assign(n, 0.0);
+  for (int32_t i = 0; i < n; ++i) {
+    ammsb_ctx* c = nullptr;
+    try {
+      c = new ammsb_ctx();
+    } catch (...) {
+      set_error("ammsb_batch_create: out of host memory");
+      return fail(AMMSB_ERR_OOM);
+    }
+    c->stream = b->stream;
+    c->own_stream = false;
+    c->owner = b;
+    c->max_ring_slots = 1;  // every iteration samples its own minibatch
+    b->chains.push_back(c);
+    const int rc = create_impl(&opts[i], &graphs[i], splits ? &splits[i] : nullptr, c);
+    if (rc != AMMSB_OK) {
+      set_error("ammsb_batch_create: chain " + std::to_string(i) + ": " + get_error());
+      return fail(rc);
+    }
+    if (!c->nbr_use_lds) {
+      set_error("ammsb_batch_create: chain " + std::to_string(i) +
+                ": num_node_sample too large for the batched neighbour sampler");
+      return fail(AMMSB_ERR_ARG);
+    }
+    sargs[i] = sampler_args(c, -1);
+    nargs[i] = neighbor_args(c);
+    pargs[i] = phi_args(c);
+    bargs[i] = beta_args(c);
+    targs[i] = theta_args(c);
+    beta_cfg[i] = batch_cfg_default(c->K, c->prec);
+    b->shapes[i].cfg = beta_cfg[i];
+    b->shapes[i].npb = default_npb(c, beta_cfg[i]);
+    if (c->mbtable_in_lds) b->mb_lds = std::max(b->mb_lds, (size_t)c->mbtable_cap * sizeof(int2));
+    b->nb_lds = std::max(b->nb_lds, (size_t)4 * c->nbr_tcap * sizeof(int2));
+    b->max_node_cap = std::max<int32_t>(b->max_node_cap, (int32_t)c->node_cap);
+    b->max_beta_blocks = std::max(b->max_beta_blocks, c->beta_blocks);
+    b->max_K = std::max(b->max_K, c->K);
+  }
+  int rc;
+  if ((rc = dalloc(&b->d_sargs, n)) || (rc = dalloc(&b->d_nargs, n)) ||
+      (rc = dalloc(&b->d_pargs, n)) || (rc = dalloc(&b->d_bargs, n)) ||
+      (rc = dalloc(&b->d_targs, n)) || (rc = dalloc(&b->d_shapes, n)) ||
+      (rc = dalloc(&b->d_beta_cfg, n)) || (rc = dalloc(&b->d_index, 2 * (n + 1) + 1)) ||
+      (rc = dalloc(&b->d_tune_index, 2 * 2 + 1)))
+    return fail(rc);
+  const struct {
+    void* d;
+    const void* h;
+    size_t bytes;
+  } up[] = {{b->d_sargs, sargs.data(), sizeof(SamplerArgs) * n},
+            {b->d_nargs, nargs.data(), sizeof(NeighborArgs) * n},
+            {b->d_pargs, pargs.data(), sizeof(PhiArgs) * n},
+            {b->d_bargs, bargs.data(), sizeof(BetaArgs) * n},
+            {b->d_targs, targs.data(), sizeof(ThetaArgs) * n},
+            {b->d_beta_cfg, beta_cfg.data(), sizeof(int32_t) * n}};
+  for (const auto& u : up)
+    if (hipMemcpy(u.d, u.h, u.bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("ammsb_batch_create: upload of kernel arguments failed");
+      return fail(AMMSB_ERR_HIP);
+    }
+  if ((rc = batch_upload_shapes(b)) != AMMSB_OK) return fail(rc);
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
+    cus = 256;
+  b->phi_grid = std::max(1, cus * 8);
+  b->finish_grid = std::max(1, cus * 4);
+  *out = b;
+  return AMMSB_OK;
+}
+
+int ammsb_batch_destroy(ammsb_batch* b) {
+  free_batch(b);
+  return AMMSB_OK;
+}
+
+int ammsb_batch_chain(ammsb_batch* b, int32_t i, ammsb_ctx** chain) {
+  if (!b || !chain || i < 0 || i >= b->n) return AMMSB_ERR_ARG;
+  *chain = b->chains[i];
+  return AMMSB_OK;
+}
+
+int ammsb_batch_init_state(ammsb_batch* b, const uint64_t* seeds) {
+  if (!b) return AMMSB_ERR_ARG;
+  for (int32_t i = 0; i < b->n; ++i) {
+    ammsb_ctx* c = b->chains[i];
+    const uint64_t seed = seeds ? seeds[i] : (uint64_t)(uint32_t)c->opt.random_seed;
+    const int rc = ammsb_init_state(c, seed);
+    if (rc != AMMSB_OK) return rc;
+  }
+  return AMMSB_OK;
+}
+
+int ammsb_batch_use_graph(ammsb_batch* b, int32_t enable) {
+  if (!b) return AMMSB_ERR_ARG;
+  b->use_graph = enable != 0;
+  return AMMSB_OK;
+}
+
+int ammsb_batch_step(ammsb_batch* b, int64_t iters) {
+  if (!b || iters < 0) return AMMSB_ERR_ARG;
+  for (auto* c : b->chains)
+    if (!c->state_ready) {
+      set_error("ammsb_batch_step: state not initialised (ammsb_batch_init_state)");
+      return AMMSB_ERR_STATE;
+    }
+  HIP_TRY(hipSetDevice(b->device));
+  // the chains' own (single-context) work must be done before the replays
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (b->use_graph && !b->exec) {
+    const int rc = batch_capture(b);
+    if (rc != AMMSB_OK) return rc;
+  }
+  for (int64_t i = 0; i < iters; ++i) {
+    if (b->use_graph) {
+      HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    } else {
+      const int rc = batch_enqueue_iteration(b);
+      if (rc != AMMSB_OK) return rc;
+    }
+  }
+  for (auto* c : b->chains) {
+    c->step_count += iters;
+    c->ring_valid = false;  // ring slot 0 holds the batch's last draws
+  }
+  return batch_check_states(b);
+}
+
+int ammsb_batch_perplexity(ammsb_batch* b, double* out) {
+  if (!b || !out) return AMMSB_ERR_ARG;
+  for (int32_t i = 0; i < b->n; ++i) {
+    const int rc = ammsb_perplexity(b->chains[i], &out[i]);
+    if (rc != AMMSB_OK) return rc;
+  }
+  return AMMSB_OK;
+}
+
+int ammsb_batch_set_shape(ammsb_batch* b, int32_t i, const ammsb_batch_shape* shape) {
+  if (!b || !shape || i < 0 || i >= b->n) return AMMSB_ERR_ARG;
+  const int cfg = shape_cfg(*shape);
+  if (cfg < 0 || !shape_valid(b->chains[i], cfg, shape->nodes_per_block)) {
+    set_error("ammsb_batch_set_shape: shape not available for this chain's K");
+    return AMMSB_ERR_ARG;
+  }
+  b->shapes[i].cfg = cfg;
+  b->shapes[i].npb = shape->nodes_per_block;
+  b->tuned_us[i] = 0.0;
+  return batch_upload_shapes(b);  // the graph reads shapes from device memory
+}
+
+int ammsb_batch_get_shape(ammsb_batch* b, int32_t i, ammsb_batch_shape* shape) {
+  if (!b || !shape || i < 0 || i >= b->n) return AMMSB_ERR_ARG;
+  shape->lanes_per_row = kBatchCfgLpr[b->shapes[i].cfg];
+  shape->vectors_per_lane = kBatchCfgCh[b->shapes[i].cfg];
+  shape->nodes_per_block = b->shapes[i].npb;
+  shape->us = b->tuned_us[i];
+  return AMMSB_OK;
+}
+
+int ammsb_batch_autotune(ammsb_batch* b, int32_t reps, ammsb_batch_shape* out) {
+  if (!b || reps < 1) return AMMSB_ERR_ARG;
+  for (auto* c : b->chains)
+    if (!c->state_ready) {
+      set_error("ammsb_batch_autotune: state not initialised");
+      return AMMSB_ERR_STATE;
+    }
+  HIP_TRY(hipSetDevice(b->device));
+  // this iteration's minibatches and neighbour sets in every chain's ring
+  int rc = batch_enqueue_samplers(b);
+  if (rc != AMMSB_OK) return rc;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const BatchIndex tix = batch_index(b->d_tune_index, 1);
+  for (int32_t i = 0; i < b->n && rc == AMMSB_OK; ++i) {
+    ammsb_ctx* c = b->chains[i];
+    double best = 1e300;
+    BatchPhiShape pick = b->shapes[i];
+    for (int cfg = 0; cfg < kBatchCfgs && rc == AMMSB_OK; ++cfg) {
+      if (!batch_cfg_valid(cfg, c->K, c->prec)) continue;
+      const int groups = kBatchWaves * (64 / kBatchCfgLpr[cfg]);
+      for (int npb = 1; npb <= groups && rc == AMMSB_OK; npb *= 2) {
+        b->shapes[i] = BatchPhiShape{cfg, npb};
+        if ((rc = batch_upload_shapes(b)) != AMMSB_OK) break;
+        // update_phi of chain i alone; it writes only the staging rows
+        auto run = [&]() -> hipError_t {
+          hipError_t e = launch_batch_index(b->d_pargs + i, b->d_shapes + i, 1, tix, b->stream);
+          if (e == hipSuccess)
+            e = launch_update_phi_batch(b->d_pargs + i, b->d_shapes + i, 1, tix, b->prec,
+                                        b->phi_grid, b->stream);
+          return e;
+        };
+        hipError_t e = run();  // warm-up
+        if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = run();
+        if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+          set_error(std::string("ammsb_batch_autotune: ") + hipGetErrorString(e));
+          rc = AMMSB_ERR_HIP;
+          break;
+        }
+        const double us = 1e3 * ms / reps;
+        if (us < best) {
+          best = us;
+          pick = b->shapes[i];
+        }
+      }
+    }
+    b->shapes[i] = pick;
+    b->tuned_us[i] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != AMMSB_OK) return rc;
+  rc = batch_upload_shapes(b);
+  if (rc != AMMSB_OK) return rc;
+  if (out)
+    for (int32_t i = 0; i < b->n; ++i) ammsb_batch_get_shape(b, i, &out[i]);
   return AMMSB_OK;
 }
 

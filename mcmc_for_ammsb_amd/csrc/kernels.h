@@ -271,4 +271,49 @@ hipError_t launch_sum(const double* x, int64_t n, double* out, hipStream_t s);
 hipError_t launch_theta_grad(const double* dbeta_partials, int32_t blocks, int32_t K,
                              const double* theta, const double* gout, double* grad_theta,
                              hipStream_t s);
+// ---------------------------------------------------------------------------
+// Batched chains: one launch per iteration phase covers every chain of a
+// batch; each kernel reads its chain's arguments from a device array.
+constexpr int kBatchWaves = 4;       // wavefronts per workgroup of the batched update_phi
+constexpr int kMaxBatchChains = 1024;
+// Row shapes of the batched update_phi and update_beta: (LPR, CH) =
+// lanes per row and 16-byte vectors per lane.
+constexpr int kBatchCfgs = 10;
+constexpr int kBatchCfgLpr[kBatchCfgs] = {4, 8, 16, 32, 64, 4, 8, 16, 32, 64};
+constexpr int kBatchCfgCh[kBatchCfgs] = {1, 1, 1, 1, 1, 2, 2, 2, 2, 2};
+// Per-chain launch shape of the batched update_phi: cfg indexes the tables
+// above, npb = minibatch nodes per workgroup (power of two, at most the
+// workgroup's kBatchWaves * 64/LPR row groups).
+struct BatchPhiShape {
+  int32_t cfg;
+  int32_t npb;
+};
+// Per-iteration offsets: items[c] = first work item of chain c (items[n] =
+// total), nodes[c] likewise for minibatch nodes; counter = work queue head.
+struct BatchIndex {
+  int32_t* items;
+  int32_t* nodes;
+  int32_t* counter;
+};
+hipError_t launch_minibatch_batch(const SamplerArgs* args, int n, size_t lds, hipStream_t s);
+hipError_t launch_neighbors_batch(const NeighborArgs* args, int n, int max_node_cap, size_t lds,
+                                  hipStream_t s);
+hipError_t launch_batch_index(const PhiArgs* args, const BatchPhiShape* shapes, int n,
+                              BatchIndex ix, hipStream_t s);
+hipError_t launch_update_phi_batch(const PhiArgs* args, const BatchPhiShape* shapes, int n,
+                                   BatchIndex ix, int precision, int grid, hipStream_t s);
+hipError_t launch_scatter_batch(const ScatterArgs* args, int n, BatchIndex ix, int precision,
+                                int grid, hipStream_t s);
+// update_phi of the batch in two launches: the persistent gather writes each
+// node's gradient sums to its staging row, the finish applies the phi step
+// and writes pi in place
+hipError_t launch_phi_finish_batch(const PhiArgs* args, int n, BatchIndex ix, int precision,
+                                   int grid, hipStream_t s);
+hipError_t launch_beta_batch(const BetaArgs* args, const int32_t* cfg, int n, int max_blocks,
+                             int precision, hipStream_t s);
+hipError_t launch_theta_batch(const ThetaArgs* args, int n, int max_K, hipStream_t s);
+// Batched update_phi shape validity for K / precision.
+bool batch_cfg_valid(int cfg, int32_t K, int precision);
+int batch_cfg_default(int32_t K, int precision);  // -1: K too large for batching
+
 }  // namespace ammsb

@@ -271,7 +271,7 @@ __device__ __forceinline__ int src_first(const int2* tab, int32_t v) {
 // the first occurrence of its node); a single thread then advances |S| over
 // the 16 counts and applies the stop rule, and the kept edges are written at
 // their sequential offsets.
-__global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
+__device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int slot) {
   extern __shared__ int2 s_dyn[];  // non-link de-dup table when table_in_lds
   __shared__ int s_wave[kMbWaves];
   __shared__ int s_row[kRowLds];    // non-links: row of the node; links: degrees
@@ -282,7 +282,6 @@ __global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
   __shared__ int s_nodes, s_total, s_used, s_err;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   DevState* st = a.st;
-  const int slot = blockIdx.x;
   const int64_t t0 = a.t0 >= 0 ? a.t0 : st->step;
   const int64_t t = t0 + slot;
   const uint32_t t32 = (uint32_t)t;
@@ -503,6 +502,10 @@ __global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
   }
 }
 
+__global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
+  minibatch_body(a, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Neighbour sampler: one wavefront per minibatch node.  Sequential
 // definition: attempt j = 0,1,...: v = u64(NEIGHBOR; j,node,t) % N, accepted
@@ -511,12 +514,12 @@ __global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
 // Output order = acceptance order; bit 31 flags a link (the y_ab of
 // update_phi, mcmc_sampler_stochastic.cc:279-283).
 template <int WAVES>
-__global__ __launch_bounds__(WAVES * 64) void k_neighbors(NeighborArgs a, int32_t slot_offset,
-                                                          int32_t slot_end, int32_t rslot0) {
+__device__ __forceinline__ void neighbors_body(const NeighborArgs& a, int32_t slot_offset,
+                                               int32_t slot_end, int32_t rslot0, int bx, int by) {
   extern __shared__ int2 s_tab[];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  const int rslot = rslot0 + blockIdx.y;  // ring slot (iteration)
-  const int slot = slot_offset + blockIdx.x * WAVES + w;  // node slot
+  const int rslot = rslot0 + by;  // ring slot (iteration)
+  const int slot = slot_offset + bx * WAVES + w;  // node slot
   const IterMeta& mt = a.ring.meta[rslot];
   const int count = mt.node_count;
   if (slot >= count || slot >= slot_end) return;
@@ -562,6 +565,12 @@ __global__ __launch_bounds__(WAVES * 64) void k_neighbors(NeighborArgs a, int32_
   }
 }
 
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void k_neighbors(NeighborArgs a, int32_t slot_offset,
+                                                          int32_t slot_end, int32_t rslot0) {
+  neighbors_body<WAVES>(a, slot_offset, slot_end, rslot0, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // update_phi + fused update_pi (one workgroup per minibatch node).
 // For node i with neighbours b (label y_b):
@@ -576,6 +585,65 @@ __global__ __launch_bounds__(WAVES * 64) void k_neighbors(NeighborArgs a, int32_
 // Waves split the neighbours; each row group streams whole neighbour rows,
 // reduces Z_b across its lanes and accumulates x_bk/Z_b - 1 per lane; the
 // waves' partial gradients meet in LDS.
+// phi step of one node (mcmc_sampler_stochastic_distr.cc:1207-1226) from the
+// row group's accumulated gradient sums acc (x_bk/Z_b - 1 summed over the
+// neighbours), written with its renormalised pi and phi_sum to stg (K+1).
+template <typename F, int LPR, int CH>
+__device__ __forceinline__ void phi_epilogue(const PhiArgs& a, int slot, int32_t node,
+                                             const F* prow, const F* pii, const F* acc, int lg,
+                                             F* stg) {
+  constexpr int V = VT<F>::V;
+  constexpr int EPL = V * CH;
+  const int K = a.K;
+  const int64_t t = a.st->step;
+  const double eps_t_d = a.eps_override >= 0.0 ? a.eps_override : eps_of_step(a.hp, t);
+  const F eps_t = (F)eps_t_d;
+  const F half_eps = eps_t / (F)2;
+  const F alpha = (F)a.hp.alpha;
+  const F Nn = (F)a.hp.Nn;
+  const F phi_sum = prow[K];
+  const F* noise = (const F*)a.noise;
+  F phin[EPL];
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+#pragma unroll
+    for (int v = 0; v < V; v += 2) {
+      const int k = (c * LPR + lg) * V + v;
+      F xi0 = (F)0, xi1 = (F)0;
+      if (k < K) {
+        if (noise) {
+          xi0 = noise[(int64_t)slot * K + k];
+          if (k + 1 < K) xi1 = noise[(int64_t)slot * K + k + 1];
+        } else {
+          normal_pair(draw4(a.key, (uint32_t)(k >> 1), (uint32_t)node, (uint32_t)t, TAG_PHI_NOISE),
+                      &xi0, &xi1);
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int j = c * V + v + h;
+        const F phik = pii[j] * phi_sum;
+        const F grad = acc[j] / phi_sum;
+        F val = fabs(phik + half_eps * (alpha - phik + Nn * grad) + sqrt(eps_t * phik) * (h ? xi1 : xi0));
+        if (val < (F)1e-24) val = (F)1e-24;
+        if (k + h >= K) val = (F)0;
+        phin[j] = val;
+        s += (double)val;
+      }
+    }
+  }
+  s = group_sum<LPR>(s);
+#pragma unroll
+  for (int c = 0; c < CH; ++c)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int k = (c * LPR + lg) * V + v;
+      if (k < K) stg[k] = (F)((double)phin[c * V + v] / s);
+    }
+  if (lg == 0) stg[K] = (F)s;
+}
+
 template <typename F, int LPR, int CH, int WAVES, int U>
 __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
   constexpr int V = VT<F>::V;
@@ -686,60 +754,13 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
   }
   if (g != 0) return;
 
-  // epilogue: phi step (mcmc_sampler_stochastic_distr.cc:1207-1226)
-  const int64_t t = a.st->step;
-  const double eps_t_d = a.eps_override >= 0.0 ? a.eps_override : eps_of_step(a.hp, t);
-  const F eps_t = (F)eps_t_d;
-  const F half_eps = eps_t / (F)2;
-  const F alpha = (F)a.hp.alpha;
-  const F Nn = (F)a.hp.Nn;
-  const F phi_sum = prow[K];
-  const F* noise = (const F*)a.noise;
-  F phin[EPL];
-  double s = 0.0;
-#pragma unroll
-  for (int c = 0; c < CH; ++c) {
-#pragma unroll
-    for (int v = 0; v < V; v += 2) {
-      const int k = (c * LPR + lg) * V + v;
-      F xi0 = (F)0, xi1 = (F)0;
-      if (k < K) {
-        if (noise) {
-          xi0 = noise[(int64_t)slot * K + k];
-          if (k + 1 < K) xi1 = noise[(int64_t)slot * K + k + 1];
-        } else {
-          normal_pair(draw4(a.key, (uint32_t)(k >> 1), (uint32_t)node, (uint32_t)t, TAG_PHI_NOISE),
-                      &xi0, &xi1);
-        }
-      }
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int j = c * V + v + h;
-        const F phik = pii[j] * phi_sum;
-        const F grad = acc[j] / phi_sum;
-        F val = fabs(phik + half_eps * (alpha - phik + Nn * grad) + sqrt(eps_t * phik) * (h ? xi1 : xi0));
-        if (val < (F)1e-24) val = (F)1e-24;
-        if (k + h >= K) val = (F)0;
-        phin[j] = val;
-        s += (double)val;
-      }
-    }
-  }
-  s = group_sum<LPR>(s);
-  F* stg = (F*)a.staging + (int64_t)slot * a.ld;
-#pragma unroll
-  for (int c = 0; c < CH; ++c)
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const int k = (c * LPR + lg) * V + v;
-      if (k < K) stg[k] = (F)((double)phin[c * V + v] / s);
-    }
-  if (lg == 0) stg[K] = (F)s;
+  phi_epilogue<F, LPR, CH>(a, slot, node, prow, pii, acc, lg,
+                           (F*)a.staging + (int64_t)slot * a.ld);
 }
 
 // Publish staged rows: pi[nodes[slot]] = staging[slot] (K+1 elements).
 template <typename F>
-__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
+__device__ __forceinline__ void scatter_body(const ScatterArgs& a, const int slot) {
   int count;
   const int32_t* nodes;
   if (a.use_ring) {
@@ -750,7 +771,6 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
     count = a.count_host;
     nodes = a.nodes;
   }
-  const int slot = blockIdx.x;
   if (slot >= count) return;
   constexpr int V = VT<F>::V;
   const int nv = (a.K + 1 + V - 1) / V;
@@ -761,6 +781,11 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
     reinterpret_cast<T*>(dst)[i] = reinterpret_cast<const T*>(src)[i];
 }
 
+template <typename F>
+__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
+  scatter_body<F>(a, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // update_beta gradient (mcmc_sampler_stochastic.cc:212-240): per edge
 //   f_k = pi_ak pi_bk, probs_k = beta_k f_k (y=1) or (1-beta_k) f_k (y=0)
@@ -769,7 +794,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
 //   G_k1 += probs_k/Z (y/theta_k1 - 1/S_k),   S_k = theta_k0 + theta_k1
 // accumulated in double; one partial [2][K] per workgroup.
 template <typename F, int LPR, int CH, int WAVES>
-__global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
+__device__ __forceinline__ void beta_body(const BetaArgs& a, const int bx) {
   constexpr int V = VT<F>::V;
   constexpr int EPL = V * CH;
   constexpr int RPW = 64 / LPR;
@@ -811,7 +836,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
   // four rows are requested, then the edges are reduced.
   constexpr int UE = CH <= 4 ? 2 : 1;  // big rows: one edge at a time (registers)
   const int gstride = a.blocks * WAVES * RPW;
-  for (int e0 = (blockIdx.x * WAVES + w) * RPW + g; e0 < count; e0 += UE * gstride) {
+  for (int e0 = (bx * WAVES + w) * RPW + g; e0 < count; e0 += UE * gstride) {
     int32_t ea[UE], eb[UE];
     int yv[UE];
     bool val[UE];
@@ -897,7 +922,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
     }
   }
   if (w != 0 || g != 0) return;
-  double* out = a.partials + (int64_t)blockIdx.x * 2 * K;
+  double* out = a.partials + (int64_t)bx * 2 * K;
 #pragma unroll
   for (int c = 0; c < CH; ++c)
 #pragma unroll
@@ -915,18 +940,23 @@ __global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
     }
 }
 
+template <typename F, int LPR, int CH, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
+  beta_body<F, LPR, CH, WAVES>(a, blockIdx.x);
+}
+
 // theta step (mcmc_sampler_stochastic.cc:245-266).  Workgroup b owns
 // k in [64b, 64b+64); its 16 wavefronts sum strided partial rows, then the
 // 16 sums meet in LDS in a fixed order (deterministic).  The last workgroup
 // to finish advances step_count, so every workgroup reads the same step.
 constexpr int kThetaThreads = 1024;
-__global__ __launch_bounds__(kThetaThreads) void k_theta(ThetaArgs a) {
+__device__ __forceinline__ void theta_body(const ThetaArgs& a, const int bx, const int gx) {
   __shared__ double s_g[kThetaThreads / 64][2][64];
   DevState* st = a.st;
   const int64_t t = st->step;
   const int tid = threadIdx.x, kl = tid & 63, rg = tid >> 6;
   const int K = a.K;
-  const int k = blockIdx.x * 64 + kl;
+  const int k = bx * 64 + kl;
   double G0 = 0.0, G1 = 0.0;
   if (k < K) {
 #pragma unroll 8
@@ -974,7 +1004,7 @@ __global__ __launch_bounds__(kThetaThreads) void k_theta(ThetaArgs a) {
     __syncthreads();
     if (tid == 0) {
       __threadfence();
-      if (atomicAdd(&st->done_count, 1) == (int)gridDim.x - 1) {
+      if (atomicAdd(&st->done_count, 1) == gx - 1) {
         const IterMeta& mt = a.meta[ring_slot(st)];
         st->nodes_total += mt.node_count;
         st->edges_total += mt.mb_count;
@@ -983,6 +1013,10 @@ __global__ __launch_bounds__(kThetaThreads) void k_theta(ThetaArgs a) {
       }
     }
   }
+}
+
+__global__ __launch_bounds__(kThetaThreads) void k_theta(ThetaArgs a) {
+  theta_body(a, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1896,6 +1930,415 @@ hipError_t launch_theta_grad(const double* part, int32_t blocks, int32_t K, cons
                              const double* gout, double* grad, hipStream_t s) {
   hipLaunchKernelGGL(k_theta_grad, dim3((K + 63) / 64), dim3(1024), 0, s, part, blocks, K, theta,
                      gout, grad);
+  return hipGetLastError();
+}
+
+}  // namespace ammsb
+
+// ===========================================================================
+// Batched chains.  Every phase of an iteration is one launch over all chains
+// of a batch (a chain = one independent sampler with its own graph, state,
+// K and seed); kernels read their chain's arguments from a device array, so
+// the launch sequence of an iteration is fixed and is captured once as a
+// hipGraph.  update_phi is a persistent kernel fed by a work queue, because
+// the chains' minibatches differ in size and cost.
+namespace ammsb {
+namespace {
+
+bool shape_rows_fit(int lpr, int ch, int32_t K, int precision) {
+  const int V = precision == 0 ? 2 : 4;
+  const int vecs = (K + V - 1) / V;
+  // every vector of the row is covered, and no configuration with half the
+  // lanes would cover it too (no all-idle lanes)
+  return lpr * ch >= vecs && (lpr == 4 || (lpr / 2) * ch < vecs);
+}
+
+// Chain c of an item: the largest c with off[c] <= item.
+__device__ __forceinline__ int chain_of(const int32_t* off, int n, int item) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= item)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Sum over the lpr lanes of a row group with lpr a uniform runtime value
+// (the same butterfly and order as group_sum<LPR>).
+template <typename T>
+__device__ __forceinline__ T group_sum_rt(T v, int lpr) {
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  if (lpr >= 8) v += dpp_mov<0x141>(v);
+  if (lpr >= 16) v += dpp_mov<0x140>(v);
+  if (lpr >= 32) {
+    const T r0 = read_lane(v, 0), r1 = read_lane(v, 16), r2 = read_lane(v, 32),
+            r3 = read_lane(v, 48);
+    if (lpr == 32)
+      v = ((int)(threadIdx.x & 63) < 32) ? r0 + r1 : r2 + r3;
+    else
+      v = (r0 + r1) + (r2 + r3);
+  }
+  return v;
+}
+
+// update_phi of npb minibatch nodes of one chain: nodes [item*npb,
+// item*npb + npb).  Rows are read by row groups of lpr lanes (a uniform
+// runtime value), CH 16-byte vectors per lane.  The workgroup's
+// G = kBatchWaves*64/lpr row groups are split evenly, gpn = G/npb groups
+// per node: gpn < 64/lpr packs several nodes into one wavefront, gpn >
+// 64/lpr spreads a node over several wavefronts.  Each group streams whole
+// neighbour rows (U in flight), reduces Z_b over its lanes and accumulates
+// x_bk/Z_b - 1; the node's groups are combined by a butterfly inside the
+// wavefront, then across its wavefronts through LDS.  The sums go to the
+// node's staging row; k_phi_finish_batch applies the phi step.
+template <typename F, int CH, int U>
+__device__ __forceinline__ void phi_batch_body(const PhiArgs& a, int item, int npb, int lpr,
+                                               F* s_red) {
+  constexpr int V = VT<F>::V;
+  constexpr int EPL = V * CH;
+  const int rpw = 64 / lpr;
+  const int G = kBatchWaves * rpw;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int g = lane / lpr, lg = lane % lpr;
+  const int gpn = G / npb;
+  const int gid = w * rpw + g;
+  const int j = gid / gpn, r = gid % gpn;
+  const int rs = ring_slot(a.st);
+  const int count = a.ring.meta[rs].node_count;
+  const int first = item * npb;  // < count: the item exists
+  const int slot = first + j;
+  const bool active = slot < count;
+  const int32_t* nodes = a.ring.nodes + (int64_t)rs * a.ring.node_cap;
+  const int32_t node = nodes[active ? slot : first];
+  const int32_t* nb = a.ring.nbrs + ((int64_t)rs * a.ring.node_cap + (active ? slot : first)) * a.nnb;
+  const F* pi = (const F*)a.pi;
+  const int K = a.K;
+  const F eps = (F)a.hp.epsilon;
+  F pii[EPL], f1[EPL], acc[EPL];
+  int koff[CH];  // first element of each of the lane's vectors (K: past the row)
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int k0 = (c * lpr + lg) * V;
+    koff[c] = k0 < K ? k0 : K;
+  }
+  const F* prow = pi + (int64_t)node * a.ld;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) load_vec<F>(prow + (koff[c] < K ? koff[c] : 0), pii + c * V);
+#pragma unroll
+  for (int c = 0; c < CH; ++c)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int k = koff[c] + v;
+      const bool in = koff[c] < K && k < K;
+      pii[c * V + v] = in ? pii[c * V + v] : (F)0;
+      f1[c * V + v] = (in ? (F)a.beta[k] : (F)0) - eps;
+      acc[c * V + v] = (F)0;
+    }
+  const F e1 = eps, e0 = (F)1 - eps;
+  const int nnb = a.nnb;
+  for (int b0 = r; b0 < nnb; b0 += U * gpn) {
+    int32_t nbv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * gpn;
+      nbv[u] = b < nnb ? nb[b] : node;  // past the end: own row, not counted
+    }
+    F row[U][EPL];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const F* rr = pi + (int64_t)(nbv[u] & 0x7fffffff) * a.ld;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) load_vec<F>(rr + (koff[c] < K ? koff[c] : 0), row[u] + c * V);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool ok = (b0 + u * gpn < nnb) && ((nbv[u] & 0x7fffffff) != node);
+      F p = (F)0;
+      // pii is zero past the row, so p needs no mask; x_bk past the row is
+      // never stored
+      if (nbv[u] < 0) {
+#pragma unroll
+        for (int jj = 0; jj < EPL; ++jj) {
+          row[u][jj] = fma(row[u][jj], f1[jj], e1);
+          p = fma(pii[jj], row[u][jj], p);
+        }
+      } else {
+#pragma unroll
+        for (int jj = 0; jj < EPL; ++jj) {
+          row[u][jj] = fma(-row[u][jj], f1[jj], e0);
+          p = fma(pii[jj], row[u][jj], p);
+        }
+      }
+      p = group_sum_rt(p, lpr);
+      const F wz = recip(p);
+      if (ok) {
+#pragma unroll
+        for (int jj = 0; jj < EPL; ++jj) acc[jj] += fma(row[u][jj], wz, (F)-1);
+      }
+    }
+  }
+  // combine the node's groups inside the wavefront: lanes of a node are
+  // contiguous, min(gpn, rpw)*lpr of them
+  const int span = (gpn < rpw ? gpn : rpw) * lpr;
+  for (int off = lpr; off < span; off <<= 1)
+#pragma unroll
+    for (int jj = 0; jj < EPL; ++jj) acc[jj] += __shfl_xor(acc[jj], off, 64);
+  bool leader = r == 0;
+  if (gpn > rpw) {  // block-uniform: the node spans wpn wavefronts
+    const int wpn = gpn / rpw;
+    if ((w % wpn) != 0 && g == 0) {
+#pragma unroll
+      for (int jj = 0; jj < EPL; ++jj) s_red[(w * EPL + jj) * 64 + lg] = acc[jj];
+    }
+    __syncthreads();
+    if ((w % wpn) == 0 && g == 0) {
+      for (int q = 1; q < wpn; ++q)
+#pragma unroll
+        for (int jj = 0; jj < EPL; ++jj) acc[jj] += s_red[((w + q) * EPL + jj) * 64 + lg];
+    }
+    leader = (w % wpn) == 0 && g == 0;
+  }
+  if (!leader || !active) return;
+  F* stg = (F*)a.staging + (int64_t)slot * a.ld;
+#pragma unroll
+  for (int c = 0; c < CH; ++c)
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      if (koff[c] + v < K) stg[koff[c] + v] = acc[c * V + v];
+}
+
+template <typename F, int U>
+__global__ __launch_bounds__(kBatchWaves * 64) void k_update_phi_batch(
+    const PhiArgs* __restrict__ args, const BatchPhiShape* __restrict__ shapes, int n,
+    BatchIndex ix) {
+  __shared__ F s_red[kBatchWaves * 8 * 64];
+  __shared__ int32_t s_off[kMaxBatchChains + 1];
+  __shared__ int s_item;
+  for (int i = threadIdx.x; i <= n; i += blockDim.x) s_off[i] = ix.items[i];
+  __syncthreads();
+  const int total = s_off[n];
+  while (true) {
+    if (threadIdx.x == 0) s_item = atomicAdd(ix.counter, 1);
+    __syncthreads();
+    // uniform values in scalar registers: the chain's arguments are then
+    // scalar loads and the shape switch a scalar branch
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (it >= total) break;
+    const int c = __builtin_amdgcn_readfirstlane(chain_of(s_off, n, it));
+    const PhiArgs& a = args[c];
+    const int cfg = __builtin_amdgcn_readfirstlane(shapes[c].cfg);
+    const int npb = __builtin_amdgcn_readfirstlane(shapes[c].npb);
+    const int local = it - __builtin_amdgcn_readfirstlane(s_off[c]);
+    const int lpr = 4 << (cfg % 5);  // the kBatchCfgLpr / kBatchCfgCh tables
+    if (cfg < 5)
+      phi_batch_body<F, 1, U>(a, local, npb, lpr, s_red);
+    else
+      phi_batch_body<F, 2, 1>(a, local, npb, lpr, s_red);
+  }
+}
+
+// Work-item and node offsets of the current iteration (one workgroup), and
+// the work queue reset: runs after the samplers, before update_phi.
+__global__ __launch_bounds__(1024) void k_batch_index(const PhiArgs* __restrict__ args,
+                                                      const BatchPhiShape* __restrict__ shapes,
+                                                      int n, BatchIndex ix) {
+  __shared__ int32_t s_items[kMaxBatchChains], s_nodes[kMaxBatchChains];
+  const int c = threadIdx.x;
+  if (c < n) {
+    const PhiArgs& a = args[c];
+    const int cnt = a.ring.meta[ring_slot(a.st)].node_count;
+    s_nodes[c] = cnt;
+    s_items[c] = (cnt + shapes[c].npb - 1) / shapes[c].npb;
+  }
+  __syncthreads();
+  if (c == 0) {
+    int32_t ti = 0, tn = 0;
+    for (int i = 0; i < n; ++i) {
+      ix.items[i] = ti;
+      ix.nodes[i] = tn;
+      ti += s_items[i];
+      tn += s_nodes[i];
+    }
+    ix.items[n] = ti;
+    ix.nodes[n] = tn;
+    *ix.counter = 0;
+  }
+}
+
+template <typename F>
+__global__ __launch_bounds__(256) void k_scatter_batch(const ScatterArgs* __restrict__ args, int n,
+                                                       BatchIndex ix) {
+  __shared__ int32_t s_off[kMaxBatchChains + 1];
+  for (int i = threadIdx.x; i <= n; i += blockDim.x) s_off[i] = ix.nodes[i];
+  __syncthreads();
+  const int total = s_off[n];
+  for (int it = blockIdx.x; it < total; it += gridDim.x) {
+    const int c = __builtin_amdgcn_readfirstlane(chain_of(s_off, n, it));
+    scatter_body<F>(args[c], it - __builtin_amdgcn_readfirstlane(s_off[c]));
+  }
+}
+
+// phi step + update_pi of every chain's minibatch nodes, one node per
+// wavefront (its first row group): reads the node's row and the gradient
+// sums staged by k_update_phi_batch and writes the new row into pi.  All
+// neighbour-row reads of the iteration are complete (previous launch), as
+// the reference refreshes pi only after the node loop
+// (mcmc_sampler_stochastic.cc:162-166).
+template <typename F, int LPR, int CH>
+__device__ __forceinline__ void phi_finish_body(const PhiArgs& a, int slot, int lg) {
+  constexpr int V = VT<F>::V;
+  constexpr int EPL = V * CH;
+  const int rs = ring_slot(a.st);
+  const int32_t node = a.ring.nodes[(int64_t)rs * a.ring.node_cap + slot];
+  F* prow = (F*)a.pi + (int64_t)node * a.ld;
+  F pii[EPL], acc[EPL];
+  load_row<F, LPR, CH>(prow, a.K, lg, pii);
+  load_row<F, LPR, CH>((const F*)a.staging + (int64_t)slot * a.ld, a.K, lg, acc);
+  phi_epilogue<F, LPR, CH>(a, slot, node, prow, pii, acc, lg, prow);
+}
+
+template <typename F>
+__global__ __launch_bounds__(256) void k_phi_finish_batch(const PhiArgs* __restrict__ args, int n,
+                                                          BatchIndex ix) {
+  __shared__ int32_t s_off[kMaxBatchChains + 1];
+  for (int i = threadIdx.x; i <= n; i += blockDim.x) s_off[i] = ix.nodes[i];
+  __syncthreads();
+  const int total = s_off[n];
+  const int lane = threadIdx.x & 63;
+  const int gw = blockIdx.x * 4 + (threadIdx.x >> 6), nw = gridDim.x * 4;
+  for (int it = gw; it < total; it += nw) {
+    const int c = __builtin_amdgcn_readfirstlane(chain_of(s_off, n, it));
+    const int slot = it - __builtin_amdgcn_readfirstlane(s_off[c]);
+    const PhiArgs& a = args[c];
+    // one wavefront per node: 64 lanes, one or two vectors each
+    if (a.K <= 64 * VT<F>::V)
+      phi_finish_body<F, 64, 1>(a, slot, lane);
+    else
+      phi_finish_body<F, 64, 2>(a, slot, lane);
+  }
+}
+
+__global__ __launch_bounds__(kMbThreads) void k_minibatch_batch(const SamplerArgs* __restrict__ args) {
+  minibatch_body(args[blockIdx.x], 0);
+}
+
+__global__ __launch_bounds__(kNbrWaves * 64) void k_neighbors_batch(
+    const NeighborArgs* __restrict__ args) {
+  neighbors_body<kNbrWaves>(args[blockIdx.y], 0, INT_MAX, 0, blockIdx.x, 0);
+}
+
+template <typename F>
+__global__ __launch_bounds__(kBetaWaves * 64) void k_beta_batch(const BetaArgs* __restrict__ args,
+                                                                const int32_t* __restrict__ cfg) {
+  const BetaArgs& a = args[blockIdx.y];
+  if ((int)blockIdx.x >= a.blocks) return;
+  switch (cfg[blockIdx.y]) {
+#define B(I)                                                                  \
+  case I:                                                                     \
+    beta_body<F, kBatchCfgLpr[I], kBatchCfgCh[I], kBetaWaves>(a, blockIdx.x); \
+    break;
+    B(0) B(1) B(2) B(3) B(4) B(5) B(6) B(7) B(8) B(9)
+#undef B
+    default:
+      break;
+  }
+}
+
+__global__ __launch_bounds__(kThetaThreads) void k_theta_batch(const ThetaArgs* __restrict__ args) {
+  const ThetaArgs& a = args[blockIdx.y];
+  const int gx = (a.K + 63) / 64;
+  if ((int)blockIdx.x >= gx) return;
+  theta_body(a, blockIdx.x, gx);
+}
+
+}  // namespace
+
+bool batch_cfg_valid(int cfg, int32_t K, int precision) {
+  if (cfg < 0 || cfg >= kBatchCfgs) return false;
+  return shape_rows_fit(kBatchCfgLpr[cfg], kBatchCfgCh[cfg], K, precision);
+}
+
+int batch_cfg_default(int32_t K, int precision) {
+  int lpr, ch;
+  if (!row_config(K, precision, &lpr, &ch) || ch > 2) return -1;
+  for (int i = 0; i < kBatchCfgs; ++i)
+    if (kBatchCfgLpr[i] == lpr && kBatchCfgCh[i] == ch) return i;
+  return -1;
+}
+
+hipError_t launch_minibatch_batch(const SamplerArgs* args, int n, size_t lds, hipStream_t s) {
+  if (n < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_minibatch_batch, dim3(n), dim3(kMbThreads), lds, s, args);
+  return hipGetLastError();
+}
+
+hipError_t launch_neighbors_batch(const NeighborArgs* args, int n, int max_node_cap, size_t lds,
+                                  hipStream_t s) {
+  if (n < 1) return hipSuccess;
+  const int blocks = (max_node_cap + kNbrWaves - 1) / kNbrWaves;
+  hipLaunchKernelGGL(k_neighbors_batch, dim3(blocks, n), dim3(kNbrWaves * 64), lds, s, args);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_index(const PhiArgs* args, const BatchPhiShape* shapes, int n,
+                              BatchIndex ix, hipStream_t s) {
+  if (n < 1 || n > kMaxBatchChains) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_batch_index, dim3(1), dim3(1024), 0, s, args, shapes, n, ix);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_phi_batch(const PhiArgs* args, const BatchPhiShape* shapes, int n,
+                                   BatchIndex ix, int precision, int grid, hipStream_t s) {
+  if (n < 1 || n > kMaxBatchChains || grid < 1) return hipErrorInvalidValue;
+  if (precision == 0)
+    hipLaunchKernelGGL((k_update_phi_batch<double, 2>), dim3(grid), dim3(kBatchWaves * 64), 0, s,
+                       args, shapes, n, ix);
+  else
+    hipLaunchKernelGGL((k_update_phi_batch<float, 2>), dim3(grid), dim3(kBatchWaves * 64), 0, s,
+                       args, shapes, n, ix);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_batch(const ScatterArgs* args, int n, BatchIndex ix, int precision,
+                                int grid, hipStream_t s) {
+  if (n < 1 || n > kMaxBatchChains || grid < 1) return hipErrorInvalidValue;
+  if (precision == 0)
+    hipLaunchKernelGGL(k_scatter_batch<double>, dim3(grid), dim3(256), 0, s, args, n, ix);
+  else
+    hipLaunchKernelGGL(k_scatter_batch<float>, dim3(grid), dim3(256), 0, s, args, n, ix);
+  return hipGetLastError();
+}
+
+hipError_t launch_phi_finish_batch(const PhiArgs* args, int n, BatchIndex ix, int precision,
+                                   int grid, hipStream_t s) {
+  if (n < 1 || n > kMaxBatchChains || grid < 1) return hipErrorInvalidValue;
+  if (precision == 0)
+    hipLaunchKernelGGL(k_phi_finish_batch<double>, dim3(grid), dim3(256), 0, s, args, n, ix);
+  else
+    hipLaunchKernelGGL(k_phi_finish_batch<float>, dim3(grid), dim3(256), 0, s, args, n, ix);
+  return hipGetLastError();
+}
+
+hipError_t launch_beta_batch(const BetaArgs* args, const int32_t* cfg, int n, int max_blocks,
+                             int precision, hipStream_t s) {
+  if (n < 1 || max_blocks < 1) return hipErrorInvalidValue;
+  if (precision == 0)
+    hipLaunchKernelGGL(k_beta_batch<double>, dim3(max_blocks, n), dim3(kBetaWaves * 64), 0, s,
+                       args, cfg);
+  else
+    hipLaunchKernelGGL(k_beta_batch<float>, dim3(max_blocks, n), dim3(kBetaWaves * 64), 0, s,
+                       args, cfg);
+  return hipGetLastError();
+}
+
+hipError_t launch_theta_batch(const ThetaArgs* args, int n, int max_K, hipStream_t s) {
+  if (n < 1 || max_K < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_theta_batch, dim3((max_K + 63) / 64, n), dim3(kThetaThreads), 0, s, args);
   return hipGetLastError();
 }
 

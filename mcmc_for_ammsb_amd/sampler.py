@@ -31,7 +31,14 @@ class MCMCSamplerStochastic:
     by the library with the reference's semantics, network.cc:503-614).
     """
 
-    def __init__(self, options, graph, split=None):
+    def __init__(self, options, graph, split=None, _borrowed=None):
+        if _borrowed is not None:  # a chain of an MCMCBatch (the batch owns it)
+            self.options, self.graph, self.split = options, graph, split
+            self._h = _borrowed
+            self._owned = False
+            self.dtype = np.float64 if options.precision == "fp64" else np.float32
+            return
+        self._owned = True
         if not isinstance(options, Options):
             options = Options.parse(options)
         if not isinstance(graph, Graph):
@@ -54,7 +61,8 @@ class MCMCSamplerStochastic:
     # ---- lifecycle -------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None):
-            _lib.lib.ammsb_destroy(self._h)
+            if getattr(self, "_owned", True):
+                _lib.lib.ammsb_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -221,3 +229,102 @@ class MCMCSamplerStochastic:
         pi, th, be = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
         _lib.check(_lib.lib.ammsb_device_ptrs(self._h, ctypes.byref(pi), ctypes.byref(th), ctypes.byref(be)))
         return pi.value, th.value, be.value
+
+
+class MCMCBatch:
+    """Many independent SGRLD chains advanced together on one GPU.
+
+    ``options``: one :class:`Options` per chain (same precision); ``graphs``
+    and ``splits`` likewise (a split of None is drawn with the reference's
+    semantics from the chain's held-out ratio and seed).  Each iteration of
+    every chain is one launch per phase over all chains, replayed from a
+    captured hipGraph.  ``chain(i)`` is a :class:`MCMCSamplerStochastic` view
+    of chain i (state, perplexity, info) owned by the batch.
+    """
+
+    def __init__(self, options, graphs, splits=None):
+        n = len(options)
+        if len(graphs) != n or (splits is not None and len(splits) != n):
+            raise ValueError("one graph (and split) per chain")
+        if splits is None:
+            splits = [None] * n
+        self.options = list(options)
+        self.graphs = list(graphs)
+        self.splits = [s if s is not None else
+                       make_split(g, o.held_out_ratio if o.held_out_ratio != 0.0 else 0.1,
+                                  o.random_seed)
+                       for o, g, s in zip(self.options, self.graphs, splits)]
+        self._opts_c = (_lib.Options * n)(*[o.to_c() for o in self.options])
+        self._g_c = (_lib.Graph * n)(*[g.to_c() for g in self.graphs])
+        self._s_c = (_lib.Split * n)(*[s.to_c() for s in self.splits])
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib.ammsb_batch_create(n, self._opts_c, self._g_c, self._s_c,
+                                               ctypes.byref(h)))
+        self._h = h
+        self.n = n
+        self._chains = []
+        for i in range(n):
+            c = ctypes.c_void_p()
+            _lib.check(_lib.lib.ammsb_batch_chain(self._h, i, ctypes.byref(c)))
+            self._chains.append(MCMCSamplerStochastic(self.options[i], self.graphs[i],
+                                                      self.splits[i], _borrowed=c))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for c in self._chains:
+                c._h = None
+            _lib.lib.ammsb_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return self.n
+
+    def chain(self, i):
+        return self._chains[i]
+
+    def init(self, seeds=None):
+        """State draw of every chain (seeds: one per chain, default each
+        chain's random_seed)."""
+        arr = None
+        if seeds is not None:
+            arr = (ctypes.c_uint64 * self.n)(*[int(x) & 0xFFFFFFFFFFFFFFFF for x in seeds])
+        _lib.check(_lib.lib.ammsb_batch_init_state(self._h, arr))
+
+    def step(self, iters=1):
+        _lib.check(_lib.lib.ammsb_batch_step(self._h, int(iters)))
+
+    def perplexity(self):
+        out = np.empty(self.n, dtype=np.float64)
+        _lib.check(_lib.lib.ammsb_batch_perplexity(self._h, _lib.ptr(out, ctypes.c_double)))
+        return out
+
+    def use_graph(self, enable=True):
+        _lib.check(_lib.lib.ammsb_batch_use_graph(self._h, 1 if enable else 0))
+
+    def get_shape(self, i):
+        sh = _lib.BatchShape()
+        _lib.check(_lib.lib.ammsb_batch_get_shape(self._h, int(i), ctypes.byref(sh)))
+        return {f: getattr(sh, f) for f, _ in _lib.BatchShape._fields_}
+
+    def set_shape(self, i, lanes_per_row, vectors_per_lane, nodes_per_block):
+        sh = _lib.BatchShape(int(lanes_per_row), int(vectors_per_lane), int(nodes_per_block), 0.0)
+        _lib.check(_lib.lib.ammsb_batch_set_shape(self._h, int(i), ctypes.byref(sh)))
+
+    def autotune(self, reps=5):
+        """Times every valid update_phi shape of each chain on its current
+        minibatch and keeps the fastest; returns the chosen shapes."""
+        out = (_lib.BatchShape * self.n)()
+        _lib.check(_lib.lib.ammsb_batch_autotune(self._h, int(reps), out))
+        return [{f: getattr(o, f) for f, _ in _lib.BatchShape._fields_} for o in out]

@@ -143,3 +143,16 @@ def test_make_pairs_invariants():
             assert (v in row) == bool(lab)
         train = {v for v in row if (min(sn, v), max(sn, v)) not in ht}
         assert train == set(nb[yy == 1].tolist())
+
+
+def test_batch_create_rejects_unsupported_chains():
+    """Argument checks of ammsb_batch_create run before any device call."""
+    g = M.generate("community", N=3000, K=16, mean_degree=20.0, seed=1)
+    s = M.make_split(g, 0.1, 42)
+    with pytest.raises(M.AmmsbError, match="exceeds the batched kernels"):
+        M.MCMCBatch([M.Options(K=1024, num_node_sample=10, precision="fp32")], [g], [s])
+    with pytest.raises(M.AmmsbError, match="share precision"):
+        M.MCMCBatch([M.Options(K=16, num_node_sample=10, precision="fp32"),
+                     M.Options(K=16, num_node_sample=10, precision="fp64")], [g, g], [s, s])
+    with pytest.raises(ValueError):
+        M.MCMCBatch([M.Options(K=16)], [g, g])
