@@ -313,6 +313,103 @@ def finite_differences(A, smp, phi, th, n_t, p_t, b_t, y_t, gphi, gth, ptr, h=1e
             "fd_theta_median_rel_err": float(np.median(errs_th))}
 
 
+def batch_workload(n_chains, seed):
+    """Chains of the batching workload: N log-uniform in [10k, 500k], K
+    cycling through {16, 64, 256, 512}, mean degree uniform in [20, 80],
+    per-chain seeds; minibatches of up to 1,024 nodes (m = 1023 non-link
+    pairs, link minibatches gathering sources up to m), 32 neighbour
+    samples, FP32, a=0.01, b=1024, c=0.55."""
+    import mcmc_for_ammsb_amd as M
+    rng = np.random.default_rng(seed)
+    ks = (16, 64, 256, 512)
+    chains = []
+    for i in range(n_chains):
+        N = int(round(np.exp(rng.uniform(np.log(10_000), np.log(500_000)))))
+        chains.append(dict(N=N, K=ks[i % len(ks)], deg=float(rng.uniform(20.0, 80.0)),
+                           seed=int(1000 + i)))
+    graphs, splits, opts = [], [], []
+    t = time.time()
+    for c in chains:
+        g = M.generate("community", N=c["N"], K=c["K"], mean_degree=c["deg"], seed=c["seed"])
+        graphs.append(g)
+        splits.append(M.make_split(g, 0.1, c["seed"]))
+        opts.append(M.Options(K=c["K"], num_node_sample=32, mini_batch_size=1023,
+                              sampler_max_source=1024, a=0.01, b=1024.0, c=0.55,
+                              random_seed=c["seed"], precision="fp32"))
+    log(f"{n_chains} chains, N {min(c['N'] for c in chains)}..{max(c['N'] for c in chains)}, "
+        f"{sum(g.num_edges for g in graphs)} edges ({time.time() - t:.1f}s)")
+    return chains, graphs, splits, opts
+
+
+def run_batch(args):
+    """Batching workload: aggregate update_phi node*K/s of all chains, for
+    the batched path (hipGraph replay, autotuned shapes) and for the same
+    chains as separate contexts on separate streams."""
+    import mcmc_for_ammsb_amd as M
+    chains, graphs, splits, opts = batch_workload(args.chains, args.seed)
+    Ks = np.array([c["K"] for c in chains], dtype=np.float64)
+    out = {"metric": "batched chains: update_phi node*K updates/s, all chains",
+           "unit": "node*K/s", "n_chains": args.chains, "steps": args.steps,
+           "warmup": args.warmup, "dtype": "f32", "data": "synthetic",
+           "workload": "N log-uniform [10k,500k], K in {16,64,256,512}, mean degree U[20,80], "
+                       "m=1023, 32 neighbour samples, FP32",
+           "chains": [{"N": c["N"], "K": c["K"], "mean_degree": round(c["deg"], 2)}
+                      for c in chains]}
+
+    def totals(views):
+        return np.array([v.totals()[0] for v in views], dtype=np.float64)
+
+    b = M.MCMCBatch(opts, graphs, splits)
+    b.init()
+    b.step(args.warmup)
+    if not args.no_autotune:
+        t = time.time()
+        shapes = b.autotune(reps=3)
+        out["autotune_s"] = time.time() - t
+        out["shapes"] = [{k: (round(v, 2) if k == "us" else v) for k, v in sh.items()}
+                         for sh in shapes]
+    views = [b.chain(i) for i in range(len(b))]
+    res = {}
+    for mode, graph in (("graph", True), ("direct", False)):
+        b.use_graph(graph)
+        b.step(2)
+        n0 = totals(views)
+        t0 = time.perf_counter()
+        b.step(args.steps)  # ends with the error check (a stream sync)
+        el = time.perf_counter() - t0
+        nk = float(((totals(views) - n0) * Ks).sum())
+        res[mode] = {"value": nk / el, "ms_per_step": 1e3 * el / args.steps}
+        log(f"batched ({mode}): {1e3 * el / args.steps:.3f} ms/step, {nk / el:.3e} node*K/s")
+    out["ppx_after"] = [round(float(x), 5) for x in b.perplexity()]
+    b.close()
+    if not args.no_separate:
+        ctxs = []
+        for o, g, s in zip(opts, graphs, splits):
+            smp = M.MCMCSamplerStochastic(o, g, s)
+            smp.init()
+            ctxs.append(smp)
+        for smp in ctxs:
+            smp.step(args.warmup, sync=False)
+        for smp in ctxs:
+            smp.step(0, sync=True)
+        n0 = totals(ctxs)
+        t0 = time.perf_counter()
+        for smp in ctxs:
+            smp.step(args.steps, sync=False)
+        for smp in ctxs:
+            smp.step(0, sync=True)
+        el = time.perf_counter() - t0
+        nk = float(((totals(ctxs) - n0) * Ks).sum())
+        res["separate_contexts"] = {"value": nk / el, "ms_per_step": 1e3 * el / args.steps}
+        log(f"separate contexts: {1e3 * el / args.steps:.3f} ms/step, {nk / el:.3e} node*K/s")
+        for smp in ctxs:
+            smp.close()
+    out["results"] = res
+    out["value"] = res["graph"]["value"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -326,7 +423,10 @@ def main():
     ap.add_argument("--phi-waves", type=int, default=0, help="update_phi wavefronts per node")
     ap.add_argument("--phi-unroll", type=int, default=0, help="update_phi rows in flight")
     ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "update_phi_traffic.json"))
-    ap.add_argument("--mode", default="throughput", choices=["throughput", "ttt", "diff"])
+    ap.add_argument("--mode", default="throughput", choices=["throughput", "ttt", "diff", "batch"])
+    ap.add_argument("--chains", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-separate", action="store_true")
     ap.add_argument("--diff-reps", type=int, default=10)
     ap.add_argument("--ttt-steps", type=int, default=3000)
     ap.add_argument("--ttt-interval", type=int, default=100)
@@ -340,6 +440,8 @@ def main():
         return run_ttt(args)
     if args.mode == "diff":
         return run_diff(args)
+    if args.mode == "batch":
+        return run_batch(args)
 
     import mcmc_for_ammsb_amd as M
     w = WORKLOADS[args.workload]

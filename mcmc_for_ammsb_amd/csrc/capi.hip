@@ -70,6 +70,7 @@ struct ammsb_ctx {
   bool own_stream = true;           // false: the stream of the batch that owns the chain
   ammsb_batch* owner = nullptr;     // batch that owns this chain, if any
   int32_t max_ring_slots = 16;
+  int64_t ring_gen = 0;             // bumped by every single-context ring refill
   Key key{};
   DevHyper hp{};
   // device graph
@@ -78,6 +79,8 @@ struct ammsb_ctx {
   uint64_t* d_hkeys = nullptr;
   uint8_t* d_hvals = nullptr;
   uint64_t hmask = 0;
+  uint64_t* d_bloom = nullptr;
+  uint64_t bloom_mask = 0;
   // held-out set (perplexity)
   int64_t H = 0, T = 0;
   int32_t* d_ho_edges = nullptr;
@@ -166,6 +169,8 @@ DevGraph dev_graph(const ammsb_ctx* c) {
   g.hkeys = c->d_hkeys;
   g.hvals = c->d_hvals;
   g.hmask = c->hmask;
+  g.bloom = c->d_bloom;
+  g.bloom_mask = c->bloom_mask;
   return g;
 }
 
@@ -323,6 +328,7 @@ ThetaArgs theta_args(const ammsb_ctx* c) {
 // ---- sample-ahead: draw `fills` iterations starting at step t0 (t0 < 0: the
 // device's step_count) into ring slots 0..fills-1
 int fill_ring(ammsb_ctx* c, int64_t t0, int32_t fills) {
+  c->ring_gen++;
   {
     PhaseScope ps(c, PH_MINIBATCH);
     HIP_TRY(launch_minibatch(sampler_args(c, t0), fills, c->stream));
@@ -444,7 +450,7 @@ void free_ctx(ammsb_ctx* c) {
                   c->d_ho_labels, c->d_R,     c->d_pi,       c->d_theta,     c->d_beta,
                   c->d_st,      c->ring.edges, c->ring.y,    c->ring.nodes,  c->ring.nbrs,
                   c->d_staging, c->d_mbtable, c->ring.stamp, c->d_nbr_gtable, c->d_beta_partials,
-                  c->ring.meta, c->d_ll_scratch,
+                  c->ring.meta, c->d_ll_scratch, c->d_bloom,
                   c->d_ppx_partials};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -581,6 +587,15 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
     hh.insert(edge_key((uint32_t)a, (uint32_t)b), (uint8_t)(2 | (tel[i] ? 4 : 0)));
   }
   c->hmask = hh.mask;
+  {
+    BloomHost bl;
+    bl.init(hc + tc);
+    for (uint64_t k : hh.keys)
+      if (k != kHashEmpty) bl.insert(k);
+    c->bloom_mask = bl.mask;
+    ALLOC_TRY(c->d_bloom, bl.words.size());
+    HIP_TRY(hipMemcpy(c->d_bloom, bl.words.data(), 8 * bl.words.size(), hipMemcpyHostToDevice));
+  }
   ALLOC_TRY(c->d_hkeys, hh.keys.size());
   ALLOC_TRY(c->d_hvals, hh.vals.size());
   HIP_TRY(hipMemcpy(c->d_hkeys, hh.keys.data(), 8 * hh.keys.size(), hipMemcpyHostToDevice));
@@ -612,6 +627,7 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
   ALLOC_TRY(c->d_st, 1);
   DevState h0{};
   h0.step = 1;
+  h0.ring_slots = 1;
   HIP_TRY(hipMemcpy(c->d_st, &h0, sizeof(DevState), hipMemcpyHostToDevice));
 
   // ---- iteration scratch (sizes: network.cc:172-201)
@@ -625,6 +641,7 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
         1, std::min<int64_t>(c->max_ring_slots, ((int64_t)1 << 30) / slot_bytes));
     Ring& R = c->ring;
     R.slots = (int32_t)slots;
+    HIP_TRY(hipMemcpy(&c->d_st->ring_slots, &R.slots, sizeof(int32_t), hipMemcpyHostToDevice));
     R.edge_cap = c->edge_cap;
     R.node_cap = c->node_cap;
     R.nnb = (int32_t)c->nnb;
@@ -643,11 +660,15 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
     // non-link de-dup table: in LDS (load <= 0.8) when it fits next to the
     // sampler's ~45 KB of static LDS, else in HBM (load <= 0.5)
     int lds_max = 0;
-    HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, o.device));
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeSharedMemPerBlockOptin, o.device) !=
+            hipSuccess ||
+        lds_max <= 0)
+      HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, o.device));
     const int64_t entries = c->m + 1024;
     const int64_t lds_cap = (int64_t)pow2_at_least((uint64_t)(entries + entries / 4 + 1));
     const int64_t static_lds = kMinibatchStaticLds;
-    if (lds_cap * (int64_t)sizeof(int2) + static_lds <= std::min<int64_t>(lds_max, 160 * 1024)) {
+    if (lds_cap * (int64_t)sizeof(int2) + static_lds <= std::min<int64_t>(lds_max, 160 * 1024) &&
+        prepare_minibatch_lds((size_t)lds_cap * sizeof(int2)) == hipSuccess) {
       c->mbtable_cap = lds_cap;
       c->mbtable_in_lds = 1;
     } else {
@@ -657,7 +678,7 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
   }
   ALLOC_TRY(c->d_mbtable, c->mbtable_in_lds ? 1 : (size_t)(c->ring.slots * c->mbtable_cap));
   c->nbr_tcap = (int32_t)pow2_at_least((uint64_t)(2 * (c->nnb + 64)));
-  c->nbr_use_lds = (4 * (int64_t)c->nbr_tcap * 8 <= 64 * 1024) ? 1 : 0;
+  c->nbr_use_lds = neighbor_lds_bytes(c->nbr_tcap) <= 64 * 1024 ? 1 : 0;
   if (!c->nbr_use_lds) {
     const int64_t budget = (int64_t)1 << 30;
     c->nbr_gslots = (int32_t)std::max<int64_t>(
@@ -1390,15 +1411,25 @@ struct ammsb_batch {
   ThetaArgs* d_targs = nullptr;
   BatchPhiShape* d_shapes = nullptr;
   int32_t* d_beta_cfg = nullptr;
-  int32_t* d_index = nullptr;     // items [n+1], nodes [n+1], counter
+  int32_t* d_index = nullptr;     // items, nodes, fin [n+1] each, counter
   int32_t* d_tune_index = nullptr;  // the same for one chain (autotuner)
+  // Sample-ahead ring of every chain: 2R slots in two halves.  Block k =
+  // iterations [ring_B + kR, ring_B + (k+1)R) lives in half k mod 2; the
+  // samplers of block k+1 run on fill_stream while block k computes.
+  int32_t R = 1;
+  hipStream_t fill_stream = nullptr;
+  hipEvent_t ev_fill[2] = {nullptr, nullptr};  // samplers of the half done
+  hipEvent_t ev_done[2] = {nullptr, nullptr};  // iterations of the half done
+  bool ring_ok = false;
+  int64_t ring_B = 0, ring_hi = 0;  // ring holds iterations [ring_hi - 2R, ring_hi)
+  std::vector<int64_t> ring_gen;    // chains' ring_gen when the ring was set up
   std::vector<BatchPhiShape> shapes;
   std::vector<double> tuned_us;
   size_t mb_lds = 0, nb_lds = 0;
   int32_t max_node_cap = 0, max_beta_blocks = 0, max_K = 0;
   int32_t phi_grid = 1, finish_grid = 1;
   bool use_graph = true;
-  hipGraph_t graph = nullptr;
+  hipGraph_t graph = nullptr;  // one iteration (samplers excluded)
   hipGraphExec_t exec = nullptr;
 };
 
@@ -1408,7 +1439,8 @@ BatchIndex batch_index(int32_t* base, int32_t n) {
   BatchIndex ix;
   ix.items = base;
   ix.nodes = base + (n + 1);
-  ix.counter = base + 2 * (n + 1);
+  ix.fin = base + 2 * (n + 1);
+  ix.counter = base + 3 * (n + 1);
   return ix;
 }
 
@@ -1423,8 +1455,14 @@ void free_batch(ammsb_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->fill_stream) (void)hipStreamSynchronize(b->fill_stream);
   batch_drop_graph(b);
   for (auto* c : b->chains) free_ctx(c);
+  for (int h = 0; h < 2; ++h) {
+    if (b->ev_fill[h]) (void)hipEventDestroy(b->ev_fill[h]);
+    if (b->ev_done[h]) (void)hipEventDestroy(b->ev_done[h]);
+  }
+  if (b->fill_stream) (void)hipStreamDestroy(b->fill_stream);
   void* ptrs[] = {b->d_sargs,  b->d_nargs,    b->d_pargs, b->d_bargs,      b->d_targs,
                   b->d_shapes, b->d_beta_cfg, b->d_index, b->d_tune_index};
   for (void* p : ptrs)
@@ -1440,23 +1478,60 @@ int batch_upload_shapes(ammsb_batch* b) {
   return AMMSB_OK;
 }
 
-// The samplers of the current iteration (ring slot 0 of every chain) and the
-// work index; no state changes (draws depend on the step only).
-int batch_enqueue_samplers(ammsb_batch* b) {
-  HIP_TRY(launch_minibatch_batch(b->d_sargs, b->n, b->mb_lds, b->stream));
-  HIP_TRY(launch_neighbors_batch(b->d_nargs, b->n, b->max_node_cap, b->nb_lds, b->stream));
-  HIP_TRY(launch_batch_index(b->d_pargs, b->d_shapes, b->n, batch_index(b->d_index, b->n),
-                             b->stream));
+// The samplers of block k (iterations ring_B + kR ..) of every chain into
+// half k mod 2 of its ring, on stream s; no state changes (draws depend on
+// the step only).
+int batch_enqueue_fill(ammsb_batch* b, int64_t k, hipStream_t s) {
+  const int64_t t0 = b->ring_B + k * b->R;
+  const int slot0 = (int)(k % 2) * b->R;
+  HIP_TRY(launch_minibatch_batch(b->d_sargs, b->n, t0, slot0, b->R, 2 * b->R, b->mb_lds, s));
+  HIP_TRY(launch_neighbors_batch(b->d_nargs, b->n, slot0, b->R, 2 * b->R, b->max_node_cap,
+                                 b->nb_lds, s));
   return AMMSB_OK;
 }
 
-int batch_enqueue_iteration(ammsb_batch* b) {
-  int rc = batch_enqueue_samplers(b);
+// Ring set-up at step S (all chains at S): ring_base = S on every chain's
+// device state, block 0 drawn on the main stream.
+int batch_ring_reset(ammsb_batch* b, int64_t S) {
+  HIP_TRY(hipStreamSynchronize(b->fill_stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  const int32_t slots = 2 * b->R;
+  for (auto* c : b->chains) {
+    HIP_TRY(hipMemcpy(&c->d_st->ring_base, &S, sizeof(int64_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(&c->d_st->ring_slots, &slots, sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  b->ring_B = S;
+  const int rc = batch_enqueue_fill(b, 0, b->stream);
   if (rc != AMMSB_OK) return rc;
+  HIP_TRY(hipEventRecord(b->ev_fill[0], b->stream));
+  b->ring_hi = S + b->R;
+  b->ring_ok = true;
+  for (int32_t i = 0; i < b->n; ++i) b->ring_gen[i] = b->chains[i]->ring_gen;
+  return AMMSB_OK;
+}
+
+// Every chain at the same step: the batch's common step, else -1.
+int64_t batch_common_step(const ammsb_batch* b) {
+  const int64_t S = b->chains[0]->step_count;
+  for (const auto* c : b->chains)
+    if (c->step_count != S) return -1;
+  return S;
+}
+
+bool batch_ring_valid(const ammsb_batch* b, int64_t S) {
+  if (!b->ring_ok || S < b->ring_hi - 2 * b->R || S >= b->ring_hi) return false;
+  for (int32_t i = 0; i < b->n; ++i)
+    if (b->chains[i]->ring_gen != b->ring_gen[i]) return false;
+  return true;
+}
+
+int batch_enqueue_iteration(ammsb_batch* b) {
   const BatchIndex ix = batch_index(b->d_index, b->n);
+  HIP_TRY(launch_batch_index(b->d_pargs, b->d_shapes, b->d_beta_cfg, b->n, ix, b->stream));
   HIP_TRY(launch_update_phi_batch(b->d_pargs, b->d_shapes, b->n, ix, b->prec, b->phi_grid,
                                   b->stream));
-  HIP_TRY(launch_phi_finish_batch(b->d_pargs, b->n, ix, b->prec, b->finish_grid, b->stream));
+  HIP_TRY(launch_phi_finish_batch(b->d_pargs, b->d_beta_cfg, b->n, ix, b->prec, b->finish_grid,
+                                  b->stream));
   HIP_TRY(launch_beta_batch(b->d_bargs, b->d_beta_cfg, b->n, b->max_beta_blocks, b->prec,
                             b->stream));
   HIP_TRY(launch_theta_batch(b->d_targs, b->n, b->max_K, b->stream));
@@ -1475,7 +1550,7 @@ int batch_capture(ammsb_batch* b) {
   }
   HIP_TRY(e);
   b->graph = g;
-  HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
   return AMMSB_OK;
 }
 
@@ -1499,11 +1574,23 @@ bool shape_valid(const ammsb_ctx* c, int cfg, int npb) {
   return npb >= 1 && npb <= groups && (npb & (npb - 1)) == 0;
 }
 
-// Default nodes per workgroup: about four neighbour rows per row group.
+// Default shape of the batched update_phi (what the autotuner picks most
+// often on chains with K in {16, 64, 256, 512}): two vectors per lane where
+// that leaves no lane idle, and 64 lanes per node for rows of up to 32
+// vectors, else 128 (two wavefronts).
+int default_cfg(const ammsb_ctx* c) {
+  for (int cfg = 0; cfg < kBatchCfgs; ++cfg)
+    if (kBatchCfgCh[cfg] == 2 && batch_cfg_valid(cfg, c->K, c->prec)) return cfg;
+  return batch_cfg_default(c->K, c->prec);
+}
+
 int default_npb(const ammsb_ctx* c, int cfg) {
-  const int groups = kBatchWaves * (64 / kBatchCfgLpr[cfg]);
-  int gpn = 1;
-  while (gpn * 2 <= groups && gpn * 2 * 4 <= c->nnb) gpn *= 2;
+  const int lpr = kBatchCfgLpr[cfg];
+  const int groups = kBatchWaves * (64 / lpr);
+  const int V = c->prec == AMMSB_FP64 ? 2 : 4;
+  const int vecs = (c->K + V - 1) / V;
+  int gpn = (vecs <= 32 ? 64 : 128) / lpr;
+  gpn = std::max(1, std::min(groups, gpn));
   return groups / gpn;
 }
 
@@ -1571,12 +1658,39 @@ int ammsb_batch_create(int32_t n, const ammsb_options* opts, const ammsb_graph* 
     c->stream = b->stream;
     c->own_stream = false;
     c->owner = b;
-    c->max_ring_slots = 1;  // every iteration samples its own minibatch
+    c->max_ring_slots = 32;  // two halves of up to 16 iterations
     b->chains.push_back(c);
     const int rc = create_impl(&opts[i], &graphs[i], splits ? &splits[i] : nullptr, c);
     if (rc != AMMSB_OK) {
       set_error("ammsb_batch_create: chain " + std::to_string(i) + ": " + get_error());
       return fail(rc);
+    }
+  }
+  // one refill cadence for the batch: every chain's ring is used as two
+  // halves of R slots, R from the smallest chain's ring
+  int32_t slots = b->chains[0]->ring.slots;
+  for (auto* c : b->chains) slots = std::min(slots, c->ring.slots);
+  b->R = std::max(1, slots / 2);
+  if (2 * b->R > slots) {
+    set_error("ammsb_batch_create: a chain's sample-ahead ring is too large for HBM budget");
+    return fail(AMMSB_ERR_OOM);
+  }
+  b->ring_gen.assign(n, -1);
+  if (hipStreamCreateWithFlags(&b->fill_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_fill[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_fill[1], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_done[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_done[1], hipEventDisableTiming) != hipSuccess) {
+    set_error("ammsb_batch_create: cannot create the sampler stream and its events");
+    return fail(AMMSB_ERR_HIP);
+  }
+  for (int32_t i = 0; i < n; ++i) {
+    ammsb_ctx* c = b->chains[i];
+    c->ring.slots = 2 * b->R;
+    if (hipMemcpy(&c->d_st->ring_slots, &c->ring.slots, sizeof(int32_t),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("ammsb_batch_create: device state update failed");
+      return fail(AMMSB_ERR_HIP);
     }
     if (!c->nbr_use_lds) {
       set_error("ammsb_batch_create: chain " + std::to_string(i) +
@@ -1589,10 +1703,10 @@ int ammsb_batch_create(int32_t n, const ammsb_options* opts, const ammsb_graph* 
     bargs[i] = beta_args(c);
     targs[i] = theta_args(c);
     beta_cfg[i] = batch_cfg_default(c->K, c->prec);
-    b->shapes[i].cfg = beta_cfg[i];
-    b->shapes[i].npb = default_npb(c, beta_cfg[i]);
+    b->shapes[i].cfg = default_cfg(c);
+    b->shapes[i].npb = default_npb(c, b->shapes[i].cfg);
     if (c->mbtable_in_lds) b->mb_lds = std::max(b->mb_lds, (size_t)c->mbtable_cap * sizeof(int2));
-    b->nb_lds = std::max(b->nb_lds, (size_t)4 * c->nbr_tcap * sizeof(int2));
+    b->nb_lds = std::max(b->nb_lds, neighbor_lds_bytes(c->nbr_tcap));
     b->max_node_cap = std::max<int32_t>(b->max_node_cap, (int32_t)c->node_cap);
     b->max_beta_blocks = std::max(b->max_beta_blocks, c->beta_blocks);
     b->max_K = std::max(b->max_K, c->K);
@@ -1601,8 +1715,8 @@ int ammsb_batch_create(int32_t n, const ammsb_options* opts, const ammsb_graph* 
   if ((rc = dalloc(&b->d_sargs, n)) || (rc = dalloc(&b->d_nargs, n)) ||
       (rc = dalloc(&b->d_pargs, n)) || (rc = dalloc(&b->d_bargs, n)) ||
       (rc = dalloc(&b->d_targs, n)) || (rc = dalloc(&b->d_shapes, n)) ||
-      (rc = dalloc(&b->d_beta_cfg, n)) || (rc = dalloc(&b->d_index, 2 * (n + 1) + 1)) ||
-      (rc = dalloc(&b->d_tune_index, 2 * 2 + 1)))
+      (rc = dalloc(&b->d_beta_cfg, n)) || (rc = dalloc(&b->d_index, 3 * (n + 1) + 1)) ||
+      (rc = dalloc(&b->d_tune_index, 3 * 2 + 1)))
     return fail(rc);
   const struct {
     void* d;
@@ -1664,25 +1778,50 @@ int ammsb_batch_step(ammsb_batch* b, int64_t iters) {
       set_error("ammsb_batch_step: state not initialised (ammsb_batch_init_state)");
       return AMMSB_ERR_STATE;
     }
+  const int64_t S = batch_common_step(b);
+  if (S < 0) {
+    set_error("ammsb_batch_step: the chains are at different steps (ammsb_set_counters)");
+    return AMMSB_ERR_STATE;
+  }
   HIP_TRY(hipSetDevice(b->device));
-  // the chains' own (single-context) work must be done before the replays
+  // the chains' own (single-context) work must be done before the batch's
   HIP_TRY(hipStreamSynchronize(b->stream));
+  if (!batch_ring_valid(b, S)) {
+    const int rc = batch_ring_reset(b, S);
+    if (rc != AMMSB_OK) return rc;
+  }
   if (b->use_graph && !b->exec) {
     const int rc = batch_capture(b);
     if (rc != AMMSB_OK) return rc;
   }
-  for (int64_t i = 0; i < iters; ++i) {
+  const int64_t R = b->R;
+  for (int64_t t = S; t < S + iters; ++t) {
+    const int64_t k = (t - b->ring_B) / R, pos = (t - b->ring_B) % R;
+    if (pos == 0) {
+      // block k's draws are complete; block k+1 is drawn on the sampler
+      // stream while block k computes, into the half block k-1 used
+      HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_fill[k % 2], 0));
+      if (b->ring_hi == b->ring_B + (k + 1) * R) {
+        HIP_TRY(hipStreamWaitEvent(b->fill_stream, b->ev_done[(k + 1) % 2], 0));
+        const int rc = batch_enqueue_fill(b, k + 1, b->fill_stream);
+        if (rc != AMMSB_OK) return rc;
+        HIP_TRY(hipEventRecord(b->ev_fill[(k + 1) % 2], b->fill_stream));
+        b->ring_hi += R;
+      }
+    }
     if (b->use_graph) {
       HIP_TRY(hipGraphLaunch(b->exec, b->stream));
     } else {
       const int rc = batch_enqueue_iteration(b);
       if (rc != AMMSB_OK) return rc;
     }
+    if (pos == R - 1) HIP_TRY(hipEventRecord(b->ev_done[k % 2], b->stream));
+    for (auto* c : b->chains) c->step_count++;
   }
-  for (auto* c : b->chains) {
-    c->step_count += iters;
-    c->ring_valid = false;  // ring slot 0 holds the batch's last draws
-  }
+  // nothing of the batch is in flight when it returns; a chain's own
+  // (single-context) ring use refills it
+  HIP_TRY(hipStreamSynchronize(b->fill_stream));
+  for (auto* c : b->chains) c->ring_valid = false;
   return batch_check_states(b);
 }
 
@@ -1726,7 +1865,13 @@ int ammsb_batch_autotune(ammsb_batch* b, int32_t reps, ammsb_batch_shape* out) {
     }
   HIP_TRY(hipSetDevice(b->device));
   // this iteration's minibatches and neighbour sets in every chain's ring
-  int rc = batch_enqueue_samplers(b);
+  const int64_t S = batch_common_step(b);
+  if (S < 0) {
+    set_error("ammsb_batch_autotune: the chains are at different steps");
+    return AMMSB_ERR_STATE;
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  int rc = batch_ring_reset(b, S);
   if (rc != AMMSB_OK) return rc;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
@@ -1744,7 +1889,8 @@ int ammsb_batch_autotune(ammsb_batch* b, int32_t reps, ammsb_batch_shape* out) {
         if ((rc = batch_upload_shapes(b)) != AMMSB_OK) break;
         // update_phi of chain i alone; it writes only the staging rows
         auto run = [&]() -> hipError_t {
-          hipError_t e = launch_batch_index(b->d_pargs + i, b->d_shapes + i, 1, tix, b->stream);
+          hipError_t e = launch_batch_index(b->d_pargs + i, b->d_shapes + i, b->d_beta_cfg + i, 1,
+                                            tix, b->stream);
           if (e == hipSuccess)
             e = launch_update_phi_batch(b->d_pargs + i, b->d_shapes + i, 1, tix, b->prec,
                                         b->phi_grid, b->stream);

@@ -23,7 +23,7 @@ struct DevState {
   int64_t nodes_total; // sum of node_count over all minibatch draws
   int64_t edges_total; // sum of mb_count over all minibatch draws
   int32_t done_count;  // k_theta workgroups finished (last one bumps step)
-  int32_t pad2;
+  int32_t ring_slots;  // slots of the sample-ahead ring
   int64_t ring_base;   // step held by slot 0 of the sample-ahead ring
 };
 
@@ -45,6 +45,8 @@ struct DevGraph {
   const uint64_t* hkeys;     // held-out ∪ test hash (host_util.h layout)
   const uint8_t* hvals;
   uint64_t hmask;
+  const uint64_t* bloom;     // blocked Bloom filter of the same keys
+  uint64_t bloom_mask;
 };
 
 // Hyper-parameters as the kernels see them (all double; each kernel casts
@@ -66,8 +68,9 @@ struct IterMeta {
 
 // Sample-ahead ring: the draws of an iteration depend only on (graph, seed,
 // step), never on the state, so `slots` future iterations are drawn by one
-// launch and consumed one per iteration.  Slot s holds iteration
-// DevState::ring_base + s.
+// launch and consumed one per iteration.  Iteration t is held by slot
+// (t - DevState::ring_base) mod slots (batches refill one half of the ring
+// while the other half is consumed).
 struct Ring {
   int32_t slots;
   int64_t edge_cap;   // edges per slot
@@ -82,7 +85,9 @@ struct Ring {
   uint32_t* stamp;    // [slots][N] node de-dup stamps
 };
 
-__device__ __forceinline__ int ring_slot(const DevState* st) { return (int)(st->step - st->ring_base); }
+__device__ __forceinline__ int ring_slot(const DevState* st) {
+  return (int)((st->step - st->ring_base) % st->ring_slots);
+}
 
 struct SamplerArgs {
   DevGraph g;
@@ -195,7 +200,16 @@ struct PpxArgs {
 // launchers (return hipError_t of the launch)
 // Draw `fills` consecutive iterations into ring slots 0..fills-1.
 hipError_t launch_minibatch(const SamplerArgs& a, int32_t fills, hipStream_t s);
+// Lets the minibatch kernels use `bytes` of dynamic LDS (above the default
+// 64 KB per workgroup where the device allows it).
+hipError_t prepare_minibatch_lds(size_t bytes);
 hipError_t launch_neighbors(const NeighborArgs& a, int32_t fills, hipStream_t s);
+// The neighbour sampler's dynamic LDS per workgroup (4 wavefronts): each
+// wavefront's de-dup table of tcap entries and a node row of kNbrRowLds.
+constexpr int kNbrRowLds = 256;
+inline size_t neighbor_lds_bytes(int32_t tcap) {
+  return (size_t)4 * ((size_t)tcap * 8 + (size_t)kNbrRowLds * 4);
+}
 hipError_t launch_update_phi(const PhiArgs& a, int precision, int32_t grid_nodes,
                              hipStream_t s, int waves, int unroll);
 // Launch shape (wavefronts per node, neighbour rows in flight) available for K.
@@ -289,17 +303,24 @@ struct BatchPhiShape {
   int32_t npb;
 };
 // Per-iteration offsets: items[c] = first work item of chain c (items[n] =
-// total), nodes[c] likewise for minibatch nodes; counter = work queue head.
+// total), nodes[c] likewise for minibatch nodes, fin[c] for the finish
+// kernel's wavefront tasks; counter = work queue head.
 struct BatchIndex {
   int32_t* items;
   int32_t* nodes;
+  int32_t* fin;
   int32_t* counter;
 };
-hipError_t launch_minibatch_batch(const SamplerArgs* args, int n, size_t lds, hipStream_t s);
-hipError_t launch_neighbors_batch(const NeighborArgs* args, int n, int max_node_cap, size_t lds,
-                                  hipStream_t s);
-hipError_t launch_batch_index(const PhiArgs* args, const BatchPhiShape* shapes, int n,
-                              BatchIndex ix, hipStream_t s);
+// Samplers of `fills` iterations ahead into ring slots 0..fills-1.
+// Iterations t0 .. t0+fills-1 of every chain into ring slots
+// (slot0 + j) mod slots; DevState::ring_base is left as it is.
+hipError_t launch_minibatch_batch(const SamplerArgs* args, int n, int64_t t0, int slot0, int fills,
+                                  int slots, size_t lds, hipStream_t s);
+hipError_t launch_neighbors_batch(const NeighborArgs* args, int n, int slot0, int fills, int slots,
+                                  int max_node_cap, size_t lds, hipStream_t s);
+// cfg: each chain's update_beta row shape, also the finish kernel's
+hipError_t launch_batch_index(const PhiArgs* args, const BatchPhiShape* shapes,
+                              const int32_t* cfg, int n, BatchIndex ix, hipStream_t s);
 hipError_t launch_update_phi_batch(const PhiArgs* args, const BatchPhiShape* shapes, int n,
                                    BatchIndex ix, int precision, int grid, hipStream_t s);
 hipError_t launch_scatter_batch(const ScatterArgs* args, int n, BatchIndex ix, int precision,
@@ -307,8 +328,8 @@ hipError_t launch_scatter_batch(const ScatterArgs* args, int n, BatchIndex ix, i
 // update_phi of the batch in two launches: the persistent gather writes each
 // node's gradient sums to its staging row, the finish applies the phi step
 // and writes pi in place
-hipError_t launch_phi_finish_batch(const PhiArgs* args, int n, BatchIndex ix, int precision,
-                                   int grid, hipStream_t s);
+hipError_t launch_phi_finish_batch(const PhiArgs* args, const int32_t* cfg, int n,
+                                   BatchIndex ix, int precision, int grid, hipStream_t s);
 hipError_t launch_beta_batch(const BetaArgs* args, const int32_t* cfg, int n, int max_blocks,
                              int precision, hipStream_t s);
 hipError_t launch_theta_batch(const ThetaArgs* args, int n, int max_K, hipStream_t s);

@@ -163,6 +163,25 @@ __device__ __forceinline__ T group_sum(T v) {
   return v;
 }
 
+// Sum over the lpr lanes of a row group with lpr a uniform runtime value
+// (the same butterfly and order as group_sum<LPR>).
+template <typename T>
+__device__ __forceinline__ T group_sum_rt(T v, int lpr) {
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  if (lpr >= 8) v += dpp_mov<0x141>(v);
+  if (lpr >= 16) v += dpp_mov<0x140>(v);
+  if (lpr >= 32) {
+    const T r0 = read_lane(v, 0), r1 = read_lane(v, 16), r2 = read_lane(v, 32),
+            r3 = read_lane(v, 48);
+    if (lpr == 32)
+      v = ((int)(threadIdx.x & 63) < 32) ? r0 + r1 : r2 + r3;
+    else
+      v = (r0 + r1) + (r2 + r3);
+  }
+  return v;
+}
+
 // Sum over the 64/LPR row groups of a wavefront (lanes with equal lg).
 template <int LPR, typename T>
 __device__ __forceinline__ T cross_group_sum(T v) {
@@ -185,6 +204,15 @@ __device__ __forceinline__ int hash_find(const DevGraph& g, uint32_t lo, uint32_
     if (k == kHashEmpty) return -1;
     s = (s + 1) & g.hmask;
   }
+}
+
+// (lo, hi) in held-out ∪ test: the Bloom word first (nearly every random
+// pair stops there), the hash only for possible members.
+__device__ __forceinline__ bool pair_held_out(const DevGraph& g, uint32_t lo, uint32_t hi) {
+  const uint64_t h = bloom_hash(edge_key(lo, hi));
+  const uint64_t bits = bloom_bits(h);
+  if ((g.bloom[h & g.bloom_mask] & bits) != bits) return false;
+  return hash_find(g, lo, hi) >= 0;
 }
 
 __device__ __forceinline__ bool row_contains(const int32_t* row, int64_t n, int32_t v) {
@@ -271,7 +299,10 @@ __device__ __forceinline__ int src_first(const int2* tab, int32_t v) {
 // the first occurrence of its node); a single thread then advances |S| over
 // the 16 counts and applies the stop rule, and the kept edges are written at
 // their sequential offsets.
-__device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int slot) {
+// Iteration t into ring slot `slot`; set_base: this slot is slot 0 of a
+// fill starting at t (DevState::ring_base = t).
+__device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int slot,
+                                               const int64_t t, const bool set_base) {
   extern __shared__ int2 s_dyn[];  // non-link de-dup table when table_in_lds
   __shared__ int s_wave[kMbWaves];
   __shared__ int s_row[kRowLds];    // non-links: row of the node; links: degrees
@@ -282,8 +313,6 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
   __shared__ int s_nodes, s_total, s_used, s_err;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   DevState* st = a.st;
-  const int64_t t0 = a.t0 >= 0 ? a.t0 : st->step;
-  const int64_t t = t0 + slot;
   const uint32_t t32 = (uint32_t)t;
   const Ring& R = a.ring;
   int32_t* const mb_edges = R.edges + (int64_t)slot * R.edge_cap * 2;
@@ -332,7 +361,7 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
       bool valid = v != node;
       const uint32_t lo = (uint32_t)min(node, v), hi = (uint32_t)max(node, v);
       if (valid) valid = !(in_lds ? row_contains(s_row, deg, v) : row_contains(a.g.cols + rs, deg, v));
-      if (valid) valid = hash_find(a.g, lo, hi) < 0;
+      if (valid) valid = !pair_held_out(a.g, lo, hi);
       int slot = -1;
       if (valid) {
         slot = table_insert(tab, cmask, v);
@@ -392,7 +421,7 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
               if (j < deg) {
                 const int32_t v = a.g.cols[rs + j];
                 keep = src_first(s_stab, v) >= s &&
-                       hash_find(a.g, (uint32_t)min(node, v), (uint32_t)max(node, v)) < 0;
+                       !pair_held_out(a.g, (uint32_t)min(node, v), (uint32_t)max(node, v));
               }
               kept += __popcll(__ballot(keep));
             }
@@ -439,7 +468,7 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
             if (j < deg) {
               v = a.g.cols[rs + j];
               keep = src_first(s_stab, v) >= s &&
-                     hash_find(a.g, (uint32_t)min(node, v), (uint32_t)max(node, v)) < 0;
+                     !pair_held_out(a.g, (uint32_t)min(node, v), (uint32_t)max(node, v));
             }
             const uint64_t mk = __ballot(keep);
             if (keep) {
@@ -498,12 +527,13 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
     mt.scale = scale;
     R.meta[slot] = mt;
     if (err) atomicOr(&st->error, err);
-    if (slot == 0) st->ring_base = t0;
+    if (set_base) st->ring_base = t;
   }
 }
 
 __global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
-  minibatch_body(a, blockIdx.x);
+  const int64_t t0 = a.t0 >= 0 ? a.t0 : a.st->step;
+  minibatch_body(a, blockIdx.x, t0 + blockIdx.x, blockIdx.x == 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -526,7 +556,8 @@ __device__ __forceinline__ void neighbors_body(const NeighborArgs& a, int32_t sl
   const uint32_t t32 = (uint32_t)mt.step;
   const uint32_t N = (uint32_t)a.g.N;
   const int32_t node = a.ring.nodes[(int64_t)rslot * a.ring.node_cap + slot];
-  int2* tab = a.use_lds ? s_tab + (size_t)w * a.tcap
+  // LDS per wavefront: the de-dup table, then room for the node's row
+  int2* tab = a.use_lds ? s_tab + (size_t)w * (a.tcap + kNbrRowLds / 2)
                         : a.gtable + (size_t)(slot - slot_offset) * a.tcap;
   for (int i = lane; i < a.tcap; i += 64) tab[i] = make_int2(-1, INT_MAX);
   __threadfence_block();
@@ -536,6 +567,15 @@ __device__ __forceinline__ void neighbors_body(const NeighborArgs& a, int32_t sl
   const int64_t deg = a.g.row_ptr[node + 1] - rs;
   const int nnb = a.ring.nnb;
   int32_t* out = a.ring.nbrs + ((int64_t)rslot * a.ring.node_cap + slot) * nnb;
+  // The node's adjacency row in LDS (up to kNbrRowLds entries) for the
+  // link test of the accepted draws; longer rows are searched in HBM.
+  const bool row_in_lds = a.use_lds && deg <= kNbrRowLds;
+  int32_t* srow = reinterpret_cast<int32_t*>(tab + a.tcap);
+  if (row_in_lds) {
+    for (int64_t q = lane; q < deg; q += 64) srow[q] = a.g.cols[rs + q];
+    __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
+  }
   int c = 0;
   const int max_batches = 4096 + 64 * ((nnb + 63) / 64);
   for (int batch = 0; c < nnb; ++batch) {
@@ -546,7 +586,7 @@ __device__ __forceinline__ void neighbors_body(const NeighborArgs& a, int32_t sl
     const uint32_t ai = (uint32_t)(batch * 64 + lane);
     const int32_t v = (int32_t)randint_mod(draw_u64(a.key, ai, (uint32_t)node, t32, TAG_NEIGHBOR), N);
     bool valid = v != node;
-    if (valid) valid = hash_find(a.g, (uint32_t)min(node, v), (uint32_t)max(node, v)) < 0;
+    if (valid) valid = !pair_held_out(a.g, (uint32_t)min(node, v), (uint32_t)max(node, v));
     int s = -1;
     if (valid) {
       s = table_insert(tab, cmask, v);
@@ -558,7 +598,7 @@ __device__ __forceinline__ void neighbors_body(const NeighborArgs& a, int32_t sl
     const uint64_t mk = __ballot(win);
     const int pos = c + __popcll(mk & ((1ull << lane) - 1ull));
     if (win && pos < nnb) {
-      const bool lk = row_contains(a.g.cols + rs, deg, v);
+      const bool lk = row_in_lds ? row_contains(srow, deg, v) : row_contains(a.g.cols + rs, deg, v);
       out[pos] = v | (lk ? (int32_t)0x80000000 : 0);
     }
     c += __popcll(mk);
@@ -588,12 +628,14 @@ __global__ __launch_bounds__(WAVES * 64) void k_neighbors(NeighborArgs a, int32_
 // phi step of one node (mcmc_sampler_stochastic_distr.cc:1207-1226) from the
 // row group's accumulated gradient sums acc (x_bk/Z_b - 1 summed over the
 // neighbours), written with its renormalised pi and phi_sum to stg (K+1).
+// LPR = 0: the row group has lpr_rt lanes (a uniform runtime value).
 template <typename F, int LPR, int CH>
 __device__ __forceinline__ void phi_epilogue(const PhiArgs& a, int slot, int32_t node,
                                              const F* prow, const F* pii, const F* acc, int lg,
-                                             F* stg) {
+                                             F* stg, int lpr_rt = 0) {
   constexpr int V = VT<F>::V;
   constexpr int EPL = V * CH;
+  const int L = LPR > 0 ? LPR : lpr_rt;
   const int K = a.K;
   const int64_t t = a.st->step;
   const double eps_t_d = a.eps_override >= 0.0 ? a.eps_override : eps_of_step(a.hp, t);
@@ -602,6 +644,7 @@ __device__ __forceinline__ void phi_epilogue(const PhiArgs& a, int slot, int32_t
   const F alpha = (F)a.hp.alpha;
   const F Nn = (F)a.hp.Nn;
   const F phi_sum = prow[K];
+  const F inv_phi_sum = recip(phi_sum);
   const F* noise = (const F*)a.noise;
   F phin[EPL];
   double s = 0.0;
@@ -609,7 +652,7 @@ __device__ __forceinline__ void phi_epilogue(const PhiArgs& a, int slot, int32_t
   for (int c = 0; c < CH; ++c) {
 #pragma unroll
     for (int v = 0; v < V; v += 2) {
-      const int k = (c * LPR + lg) * V + v;
+      const int k = (c * L + lg) * V + v;
       F xi0 = (F)0, xi1 = (F)0;
       if (k < K) {
         if (noise) {
@@ -624,7 +667,7 @@ __device__ __forceinline__ void phi_epilogue(const PhiArgs& a, int slot, int32_t
       for (int h = 0; h < 2; ++h) {
         const int j = c * V + v + h;
         const F phik = pii[j] * phi_sum;
-        const F grad = acc[j] / phi_sum;
+        const F grad = acc[j] * inv_phi_sum;
         F val = fabs(phik + half_eps * (alpha - phik + Nn * grad) + sqrt(eps_t * phik) * (h ? xi1 : xi0));
         if (val < (F)1e-24) val = (F)1e-24;
         if (k + h >= K) val = (F)0;
@@ -633,13 +676,17 @@ __device__ __forceinline__ void phi_epilogue(const PhiArgs& a, int slot, int32_t
       }
     }
   }
-  s = group_sum<LPR>(s);
+  if constexpr (LPR > 0)
+    s = group_sum<LPR>(s);
+  else
+    s = group_sum_rt(s, lpr_rt);
+  const double inv_s = 1.0 / s;
 #pragma unroll
   for (int c = 0; c < CH; ++c)
 #pragma unroll
     for (int v = 0; v < V; ++v) {
-      const int k = (c * LPR + lg) * V + v;
-      if (k < K) stg[k] = (F)((double)phin[c * V + v] / s);
+      const int k = (c * L + lg) * V + v;
+      if (k < K) stg[k] = (F)((double)phin[c * V + v] * inv_s);
     }
   if (lg == 0) stg[K] = (F)s;
 }
@@ -1245,7 +1292,7 @@ hipError_t launch_minibatch(const SamplerArgs& a, int32_t fills, hipStream_t s) 
 hipError_t launch_neighbors(const NeighborArgs& a, int32_t fills, hipStream_t s) {
   const int32_t grid_nodes = (int32_t)a.ring.node_cap;
   if (a.use_lds) {
-    const size_t lds = (size_t)kNbrWaves * a.tcap * sizeof(int2);
+    const size_t lds = neighbor_lds_bytes(a.tcap);
     const int blocks = (grid_nodes + kNbrWaves - 1) / kNbrWaves;
     hipLaunchKernelGGL(k_neighbors<kNbrWaves>, dim3(blocks, fills), dim3(kNbrWaves * 64), lds, s,
                        a, 0, grid_nodes, 0);
@@ -1966,25 +2013,6 @@ __device__ __forceinline__ int chain_of(const int32_t* off, int n, int item) {
   return lo;
 }
 
-// Sum over the lpr lanes of a row group with lpr a uniform runtime value
-// (the same butterfly and order as group_sum<LPR>).
-template <typename T>
-__device__ __forceinline__ T group_sum_rt(T v, int lpr) {
-  v += dpp_mov<0xB1>(v);
-  v += dpp_mov<0x4E>(v);
-  if (lpr >= 8) v += dpp_mov<0x141>(v);
-  if (lpr >= 16) v += dpp_mov<0x140>(v);
-  if (lpr >= 32) {
-    const T r0 = read_lane(v, 0), r1 = read_lane(v, 16), r2 = read_lane(v, 32),
-            r3 = read_lane(v, 48);
-    if (lpr == 32)
-      v = ((int)(threadIdx.x & 63) < 32) ? r0 + r1 : r2 + r3;
-    else
-      v = (r0 + r1) + (r2 + r3);
-  }
-  return v;
-}
-
 // update_phi of npb minibatch nodes of one chain: nodes [item*npb,
 // item*npb + npb).  Rows are read by row groups of lpr lanes (a uniform
 // runtime value), CH 16-byte vectors per lane.  The workgroup's
@@ -2040,19 +2068,28 @@ __device__ __forceinline__ void phi_batch_body(const PhiArgs& a, int item, int n
     }
   const F e1 = eps, e0 = (F)1 - eps;
   const int nnb = a.nnb;
+  // neighbour ids one iteration ahead, so that an iteration's row loads do
+  // not wait for its id loads
+  int32_t nxt[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int b = r + u * gpn;
+    nxt[u] = b < nnb ? nb[b] : node;  // past the end: own row, not counted
+  }
   for (int b0 = r; b0 < nnb; b0 += U * gpn) {
     int32_t nbv[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int b = b0 + u * gpn;
-      nbv[u] = b < nnb ? nb[b] : node;  // past the end: own row, not counted
-    }
     F row[U][EPL];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
+      nbv[u] = nxt[u];
       const F* rr = pi + (int64_t)(nbv[u] & 0x7fffffff) * a.ld;
 #pragma unroll
       for (int c = 0; c < CH; ++c) load_vec<F>(rr + (koff[c] < K ? koff[c] : 0), row[u] + c * V);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + (U + u) * gpn;
+      nxt[u] = b < nnb ? nb[b] : node;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -2138,7 +2175,7 @@ __global__ __launch_bounds__(kBatchWaves * 64) void k_update_phi_batch(
     if (cfg < 5)
       phi_batch_body<F, 1, U>(a, local, npb, lpr, s_red);
     else
-      phi_batch_body<F, 2, 1>(a, local, npb, lpr, s_red);
+      phi_batch_body<F, 2, U>(a, local, npb, lpr, s_red);
   }
 }
 
@@ -2146,26 +2183,33 @@ __global__ __launch_bounds__(kBatchWaves * 64) void k_update_phi_batch(
 // the work queue reset: runs after the samplers, before update_phi.
 __global__ __launch_bounds__(1024) void k_batch_index(const PhiArgs* __restrict__ args,
                                                       const BatchPhiShape* __restrict__ shapes,
-                                                      int n, BatchIndex ix) {
-  __shared__ int32_t s_items[kMaxBatchChains], s_nodes[kMaxBatchChains];
+                                                      const int32_t* __restrict__ cfg, int n,
+                                                      BatchIndex ix) {
+  __shared__ int32_t s_items[kMaxBatchChains], s_nodes[kMaxBatchChains],
+      s_fin[kMaxBatchChains];
   const int c = threadIdx.x;
   if (c < n) {
     const PhiArgs& a = args[c];
     const int cnt = a.ring.meta[ring_slot(a.st)].node_count;
+    const int rpw = 64 / (4 << (cfg[c] % 5));  // finish: row groups per wavefront
     s_nodes[c] = cnt;
     s_items[c] = (cnt + shapes[c].npb - 1) / shapes[c].npb;
+    s_fin[c] = (cnt + rpw - 1) / rpw;
   }
   __syncthreads();
   if (c == 0) {
-    int32_t ti = 0, tn = 0;
+    int32_t ti = 0, tn = 0, tf = 0;
     for (int i = 0; i < n; ++i) {
       ix.items[i] = ti;
       ix.nodes[i] = tn;
+      ix.fin[i] = tf;
       ti += s_items[i];
       tn += s_nodes[i];
+      tf += s_fin[i];
     }
     ix.items[n] = ti;
     ix.nodes[n] = tn;
+    ix.fin[n] = tf;
     *ix.counter = 0;
   }
 }
@@ -2183,53 +2227,77 @@ __global__ __launch_bounds__(256) void k_scatter_batch(const ScatterArgs* __rest
   }
 }
 
-// phi step + update_pi of every chain's minibatch nodes, one node per
-// wavefront (its first row group): reads the node's row and the gradient
-// sums staged by k_update_phi_batch and writes the new row into pi.  All
-// neighbour-row reads of the iteration are complete (previous launch), as
-// the reference refreshes pi only after the node loop
+// phi step + update_pi of every chain's minibatch nodes: a wavefront takes
+// 64/lpr consecutive nodes of one chain, one per row group of lpr lanes
+// (the chain's update_beta row shape).  Reads the node's row and the
+// gradient sums staged by k_update_phi_batch and writes the new row into
+// pi.  All neighbour-row reads of the iteration are complete (previous
+// launch), as the reference refreshes pi only after the node loop
 // (mcmc_sampler_stochastic.cc:162-166).
-template <typename F, int LPR, int CH>
-__device__ __forceinline__ void phi_finish_body(const PhiArgs& a, int slot, int lg) {
+template <typename F, int CH>
+__device__ __forceinline__ void phi_finish_body(const PhiArgs& a, int slot, int lg, int lpr) {
   constexpr int V = VT<F>::V;
   constexpr int EPL = V * CH;
   const int rs = ring_slot(a.st);
   const int32_t node = a.ring.nodes[(int64_t)rs * a.ring.node_cap + slot];
   F* prow = (F*)a.pi + (int64_t)node * a.ld;
+  const F* acc_row = (const F*)a.staging + (int64_t)slot * a.ld;
+  const int K = a.K;
   F pii[EPL], acc[EPL];
-  load_row<F, LPR, CH>(prow, a.K, lg, pii);
-  load_row<F, LPR, CH>((const F*)a.staging + (int64_t)slot * a.ld, a.K, lg, acc);
-  phi_epilogue<F, LPR, CH>(a, slot, node, prow, pii, acc, lg, prow);
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int k0 = (c * lpr + lg) * V;
+    load_vec<F>(prow + (k0 < K ? k0 : 0), pii + c * V);
+    load_vec<F>(acc_row + (k0 < K ? k0 : 0), acc + c * V);
+  }
+#pragma unroll
+  for (int c = 0; c < CH; ++c)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const bool in = (c * lpr + lg) * V + v < K;
+      pii[c * V + v] = in ? pii[c * V + v] : (F)0;
+      acc[c * V + v] = in ? acc[c * V + v] : (F)0;
+    }
+  phi_epilogue<F, 0, CH>(a, slot, node, prow, pii, acc, lg, prow, lpr);
 }
 
 template <typename F>
-__global__ __launch_bounds__(256) void k_phi_finish_batch(const PhiArgs* __restrict__ args, int n,
+__global__ __launch_bounds__(256) void k_phi_finish_batch(const PhiArgs* __restrict__ args,
+                                                          const int32_t* __restrict__ cfg, int n,
                                                           BatchIndex ix) {
   __shared__ int32_t s_off[kMaxBatchChains + 1];
-  for (int i = threadIdx.x; i <= n; i += blockDim.x) s_off[i] = ix.nodes[i];
+  for (int i = threadIdx.x; i <= n; i += blockDim.x) s_off[i] = ix.fin[i];
   __syncthreads();
   const int total = s_off[n];
   const int lane = threadIdx.x & 63;
   const int gw = blockIdx.x * 4 + (threadIdx.x >> 6), nw = gridDim.x * 4;
   for (int it = gw; it < total; it += nw) {
     const int c = __builtin_amdgcn_readfirstlane(chain_of(s_off, n, it));
-    const int slot = it - __builtin_amdgcn_readfirstlane(s_off[c]);
+    const int cf = __builtin_amdgcn_readfirstlane(cfg[c]);
+    const int lpr = 4 << (cf % 5);  // the kBatchCfgLpr / kBatchCfgCh tables
     const PhiArgs& a = args[c];
-    // one wavefront per node: 64 lanes, one or two vectors each
-    if (a.K <= 64 * VT<F>::V)
-      phi_finish_body<F, 64, 1>(a, slot, lane);
+    const int count = a.ring.meta[ring_slot(a.st)].node_count;
+    const int slot = (it - __builtin_amdgcn_readfirstlane(s_off[c])) * (64 / lpr) + lane / lpr;
+    if (slot >= count) continue;
+    if (cf < 5)
+      phi_finish_body<F, 1>(a, slot, lane % lpr, lpr);
     else
-      phi_finish_body<F, 64, 2>(a, slot, lane);
+      phi_finish_body<F, 2>(a, slot, lane % lpr, lpr);
   }
 }
 
-__global__ __launch_bounds__(kMbThreads) void k_minibatch_batch(const SamplerArgs* __restrict__ args) {
-  minibatch_body(args[blockIdx.x], 0);
+// Samplers of iterations t0 + j (j < fills) of every chain into ring slots
+// (slot0 + j) mod slots: workgroup (chain, j).
+__global__ __launch_bounds__(kMbThreads) void k_minibatch_batch(const SamplerArgs* __restrict__ args,
+                                                               int64_t t0, int slot0, int slots) {
+  const int j = blockIdx.y;
+  minibatch_body(args[blockIdx.x], (slot0 + j) % slots, t0 + j, false);
 }
 
 __global__ __launch_bounds__(kNbrWaves * 64) void k_neighbors_batch(
-    const NeighborArgs* __restrict__ args) {
-  neighbors_body<kNbrWaves>(args[blockIdx.y], 0, INT_MAX, 0, blockIdx.x, 0);
+    const NeighborArgs* __restrict__ args, int slot0, int fills, int slots) {
+  const int c = blockIdx.y / fills, j = blockIdx.y % fills;
+  neighbors_body<kNbrWaves>(args[c], 0, INT_MAX, 0, blockIdx.x, (slot0 + j) % slots);
 }
 
 template <typename F>
@@ -2271,24 +2339,27 @@ int batch_cfg_default(int32_t K, int precision) {
   return -1;
 }
 
-hipError_t launch_minibatch_batch(const SamplerArgs* args, int n, size_t lds, hipStream_t s) {
-  if (n < 1) return hipSuccess;
-  hipLaunchKernelGGL(k_minibatch_batch, dim3(n), dim3(kMbThreads), lds, s, args);
+hipError_t launch_minibatch_batch(const SamplerArgs* args, int n, int64_t t0, int slot0, int fills,
+                                  int slots, size_t lds, hipStream_t s) {
+  if (n < 1 || fills < 1 || slots < fills || slot0 < 0 || t0 < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_minibatch_batch, dim3(n, fills), dim3(kMbThreads), lds, s, args, t0, slot0,
+                     slots);
   return hipGetLastError();
 }
 
-hipError_t launch_neighbors_batch(const NeighborArgs* args, int n, int max_node_cap, size_t lds,
-                                  hipStream_t s) {
-  if (n < 1) return hipSuccess;
+hipError_t launch_neighbors_batch(const NeighborArgs* args, int n, int slot0, int fills, int slots,
+                                  int max_node_cap, size_t lds, hipStream_t s) {
+  if (n < 1 || fills < 1 || slots < fills || slot0 < 0) return hipErrorInvalidValue;
   const int blocks = (max_node_cap + kNbrWaves - 1) / kNbrWaves;
-  hipLaunchKernelGGL(k_neighbors_batch, dim3(blocks, n), dim3(kNbrWaves * 64), lds, s, args);
+  hipLaunchKernelGGL(k_neighbors_batch, dim3(blocks, n * fills), dim3(kNbrWaves * 64), lds, s,
+                     args, slot0, fills, slots);
   return hipGetLastError();
 }
 
-hipError_t launch_batch_index(const PhiArgs* args, const BatchPhiShape* shapes, int n,
-                              BatchIndex ix, hipStream_t s) {
+hipError_t launch_batch_index(const PhiArgs* args, const BatchPhiShape* shapes,
+                              const int32_t* cfg, int n, BatchIndex ix, hipStream_t s) {
   if (n < 1 || n > kMaxBatchChains) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_batch_index, dim3(1), dim3(1024), 0, s, args, shapes, n, ix);
+  hipLaunchKernelGGL(k_batch_index, dim3(1), dim3(1024), 0, s, args, shapes, cfg, n, ix);
   return hipGetLastError();
 }
 
@@ -2314,13 +2385,13 @@ hipError_t launch_scatter_batch(const ScatterArgs* args, int n, BatchIndex ix, i
   return hipGetLastError();
 }
 
-hipError_t launch_phi_finish_batch(const PhiArgs* args, int n, BatchIndex ix, int precision,
-                                   int grid, hipStream_t s) {
+hipError_t launch_phi_finish_batch(const PhiArgs* args, const int32_t* cfg, int n,
+                                   BatchIndex ix, int precision, int grid, hipStream_t s) {
   if (n < 1 || n > kMaxBatchChains || grid < 1) return hipErrorInvalidValue;
   if (precision == 0)
-    hipLaunchKernelGGL(k_phi_finish_batch<double>, dim3(grid), dim3(256), 0, s, args, n, ix);
+    hipLaunchKernelGGL(k_phi_finish_batch<double>, dim3(grid), dim3(256), 0, s, args, cfg, n, ix);
   else
-    hipLaunchKernelGGL(k_phi_finish_batch<float>, dim3(grid), dim3(256), 0, s, args, n, ix);
+    hipLaunchKernelGGL(k_phi_finish_batch<float>, dim3(grid), dim3(256), 0, s, args, cfg, n, ix);
   return hipGetLastError();
 }
 
@@ -2342,4 +2413,18 @@ hipError_t launch_theta_batch(const ThetaArgs* args, int n, int max_K, hipStream
   return hipGetLastError();
 }
 
+}  // namespace ammsb
+
+namespace ammsb {
+hipError_t prepare_minibatch_lds(size_t bytes) {
+  static size_t granted = 64 * 1024 - kMinibatchStaticLds;
+  if (bytes <= granted) return hipSuccess;
+  hipError_t e = hipFuncSetAttribute((const void*)k_minibatch,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_minibatch_batch,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) granted = bytes;
+  return e;
+}
 }  // namespace ammsb
