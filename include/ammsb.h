@@ -43,7 +43,8 @@ enum ammsb_strategy {
   AMMSB_STRATIFIED_RANDOM_NODE = 0,
   AMMSB_STRATIFIED_RANDOM_NODE_LINKS = 1,
   AMMSB_STRATIFIED_RANDOM_NODE_NONLINKS = 2,
-  AMMSB_RANDOM_EDGE = 3
+  AMMSB_RANDOM_EDGE = 3  /* random_edge_sampling (network.cc:237-268); needs
+                            2*num_edges < 2^31 */
 };
 
 /* ---- Float precision (c++/src/mcmc/config.h.in:14-20 picks float or double

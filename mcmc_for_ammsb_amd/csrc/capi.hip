@@ -81,6 +81,10 @@ struct ammsb_ctx {
   uint64_t hmask = 0;
   uint64_t* d_bloom = nullptr;
   uint64_t bloom_mask = 0;
+  int32_t* d_esrc = nullptr;                 // random-edge: row of each CSR entry
+  unsigned long long* d_re_keys = nullptr;   // random-edge de-dup tables
+  int32_t* d_re_att = nullptr;
+  int64_t re_cap = 0;
   // held-out set (perplexity)
   int64_t H = 0, T = 0;
   int32_t* d_ho_edges = nullptr;
@@ -171,6 +175,7 @@ DevGraph dev_graph(const ammsb_ctx* c) {
   g.hmask = c->hmask;
   g.bloom = c->d_bloom;
   g.bloom_mask = c->bloom_mask;
+  g.esrc = c->d_esrc;
   return g;
 }
 
@@ -236,6 +241,10 @@ SamplerArgs sampler_args(const ammsb_ctx* c, int64_t t0) {
   sa.table = c->d_mbtable;
   sa.table_cap = c->mbtable_cap;
   sa.table_in_lds = c->mbtable_in_lds;
+  sa.held_test = c->H + c->T;
+  sa.re_keys = c->d_re_keys;
+  sa.re_att = c->d_re_att;
+  sa.re_cap = c->re_cap;
   return sa;
 }
 
@@ -450,7 +459,7 @@ void free_ctx(ammsb_ctx* c) {
                   c->d_ho_labels, c->d_R,     c->d_pi,       c->d_theta,     c->d_beta,
                   c->d_st,      c->ring.edges, c->ring.y,    c->ring.nodes,  c->ring.nbrs,
                   c->d_staging, c->d_mbtable, c->ring.stamp, c->d_nbr_gtable, c->d_beta_partials,
-                  c->ring.meta, c->d_ll_scratch, c->d_bloom,
+                  c->ring.meta, c->d_ll_scratch, c->d_bloom, c->d_esrc, c->d_re_keys, c->d_re_att,
                   c->d_ppx_partials};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -483,8 +492,8 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
     set_error("ammsb_create: Invalid sampling strategy");
     return AMMSB_ERR_ARG;
   }
-  if (o.strategy == AMMSB_RANDOM_EDGE) {
-    set_error("ammsb_create: strategy random-edge is not implemented on the device path");
+  if (o.strategy == AMMSB_RANDOM_EDGE && 2 * g->num_edges > (int64_t)INT32_MAX) {
+    set_error("ammsb_create: random-edge minibatches need 2*num_edges < 2^31");
     return AMMSB_ERR_ARG;
   }
   c->N = g->N;
@@ -632,7 +641,13 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
 
   // ---- iteration scratch (sizes: network.cc:172-201)
   c->edge_cap = c->m + c->max_degree + 1;
+  // random-edge with m >= E - |held-out| - |test| takes the whole training
+  // set, which can hold more than m links (network.cc:238-240)
+  if (o.strategy == AMMSB_RANDOM_EDGE && c->m >= c->E - (c->H + c->T))
+    c->edge_cap = std::max<int64_t>(c->edge_cap, c->E + 1);
   c->node_cap = std::min<int64_t>(c->N, c->edge_cap + std::min<int64_t>(o.sampler_max_source, c->N) + 1);
+  // random edges: up to two new endpoints per edge
+  if (o.strategy == AMMSB_RANDOM_EDGE) c->node_cap = std::min<int64_t>(c->N, 2 * c->edge_cap);
   {
     // sample-ahead ring: up to 16 iterations, at most ~1 GiB
     const int64_t slot_bytes = c->edge_cap * 9 + c->node_cap * 4 + c->node_cap * c->nnb * 4 +
@@ -677,6 +692,17 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
     }
   }
   ALLOC_TRY(c->d_mbtable, c->mbtable_in_lds ? 1 : (size_t)(c->ring.slots * c->mbtable_cap));
+  if (o.strategy == AMMSB_RANDOM_EDGE) {
+    // row of every CSR entry, and the per-slot edge de-dup tables
+    std::vector<int32_t> esrc((size_t)(2 * c->E));
+    for (int32_t a = 0; a < c->N; ++a)
+      for (int64_t q = g->row_ptr[a]; q < g->row_ptr[a + 1]; ++q) esrc[(size_t)q] = a;
+    ALLOC_TRY(c->d_esrc, esrc.size());
+    HIP_TRY(hipMemcpy(c->d_esrc, esrc.data(), 4 * esrc.size(), hipMemcpyHostToDevice));
+    c->re_cap = (int64_t)pow2_at_least((uint64_t)(2 * (c->m + 1024)));
+    ALLOC_TRY(c->d_re_keys, (size_t)(c->ring.slots * c->re_cap));
+    ALLOC_TRY(c->d_re_att, (size_t)(c->ring.slots * c->re_cap));
+  }
   c->nbr_tcap = (int32_t)pow2_at_least((uint64_t)(2 * (c->nnb + 64)));
   c->nbr_use_lds = neighbor_lds_bytes(c->nbr_tcap) <= 64 * 1024 ? 1 : 0;
   if (!c->nbr_use_lds) {
@@ -1608,10 +1634,6 @@ int ammsb_batch_create(int32_t n, const ammsb_options* opts, const ammsb_graph* 
   for (int32_t i = 0; i < n; ++i) {
     if (opts[i].precision != opts[0].precision || opts[i].device != opts[0].device) {
       set_error("ammsb_batch_create: all chains must share precision and device");
-      return AMMSB_ERR_ARG;
-    }
-    if (opts[i].strategy == AMMSB_RANDOM_EDGE) {
-      set_error("ammsb_batch_create: batches use the stratified minibatch strategies");
       return AMMSB_ERR_ARG;
     }
     if (batch_cfg_default((int32_t)opts[i].K, opts[i].precision) < 0) {

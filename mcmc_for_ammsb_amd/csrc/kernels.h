@@ -47,6 +47,7 @@ struct DevGraph {
   uint64_t hmask;
   const uint64_t* bloom;     // blocked Bloom filter of the same keys
   uint64_t bloom_mask;
+  const int32_t* esrc;       // [2E] row of each CSR entry (random-edge strategy only)
 };
 
 // Hyper-parameters as the kernels see them (all double; each kernel casts
@@ -101,6 +102,12 @@ struct SamplerArgs {
   int2* table;           // non-link de-dup tables in HBM [slots][table_cap] (if not in LDS)
   int64_t table_cap;     // power of two, > 1.25 x (m + 1024) entries
   int32_t table_in_lds;  // 1: the table lives in dynamic LDS instead
+  // random-edge strategy: held-out + test entry count and the edge de-dup
+  // tables in HBM [slots][re_cap] (keys a<<32|b, smallest attempt)
+  int64_t held_test;
+  unsigned long long* re_keys;
+  int32_t* re_att;
+  int64_t re_cap;
 };
 
 struct NeighborArgs {

@@ -237,6 +237,17 @@ __device__ __forceinline__ int table_insert(int2* tab, uint32_t cap_mask, int32_
   }
 }
 
+// De-dup table of 64-bit pair keys (random-edge minibatches).
+__device__ __forceinline__ int table_insert64(unsigned long long* keys, uint32_t cap_mask,
+                                              unsigned long long k) {
+  uint32_t s = (uint32_t)mix64(k) & cap_mask;
+  while (true) {
+    const unsigned long long prev = atomicCAS(&keys[s], ~0ull, k);
+    if (prev == ~0ull || prev == k) return (int)s;
+    s = (s + 1) & cap_mask;
+  }
+}
+
 __device__ __forceinline__ int load_agent(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -330,6 +341,8 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
     flag = 1;
   else if (a.strategy == 2)
     flag = 0;
+  else if (a.strategy == 3)
+    flag = 2;
   else
     flag = -1;
   int64_t c = 0;
@@ -384,6 +397,76 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
     const double dN = (double)a.g.N, dE = (double)a.g.E;
     scale = dN * dN / (double)m;
     scale *= dE / (dN * (dN - 1.0) / 2.0 - dE);
+  } else if (flag == 2) {
+    // random edge (network.cc:237-268).  Full training set when m >=
+    // E - |held-out| - |test| (the reference's entry counts): the training
+    // links a<b in CSR order.  Else attempt j: q = u64(RANDOM_EDGE; j,0,t) %
+    // 2E, the CSR entry q (row esrc[q], column cols[q]) as (min, max),
+    // accepted iff not held-out/test and not accepted before, until m.
+    const int64_t E2 = 2 * a.g.E;
+    if (m >= a.g.E - a.held_test) {
+      for (int64_t q0 = 0; q0 < E2; q0 += kMbThreads) {
+        const int64_t q = q0 + tid;
+        bool keep = false;
+        int32_t u = 0, v = 0;
+        if (q < E2) {
+          u = a.g.esrc[q];
+          v = a.g.cols[q];
+          keep = u < v && !pair_held_out(a.g, (uint32_t)u, (uint32_t)v);
+        }
+        int total;
+        const int pos = block_scan(keep, s_wave, &total);
+        if (keep && c + pos < edge_cap) {
+          mb_edges[2 * (c + pos)] = u;
+          mb_edges[2 * (c + pos) + 1] = v;
+          mb_y[c + pos] = 1;
+        }
+        if (c + total > edge_cap) err |= ERR_MB_OVERFLOW;
+        c = min(edge_cap, c + (int64_t)total);
+      }
+    } else {
+      unsigned long long* keys = a.re_keys + (int64_t)slot * a.re_cap;
+      int32_t* att = a.re_att + (int64_t)slot * a.re_cap;
+      for (int64_t i = tid; i < a.re_cap; i += kMbThreads) {
+        keys[i] = ~0ull;
+        att[i] = INT_MAX;
+      }
+      __threadfence();
+      __syncthreads();
+      const uint32_t cmask = (uint32_t)(a.re_cap - 1);
+      const int64_t max_batches = 4096 + 64 * ((m + kMbThreads - 1) / kMbThreads);
+      for (int64_t batch = 0;; ++batch) {
+        if (batch >= max_batches) {
+          err |= ERR_NONLINK_ATTEMPTS;
+          break;
+        }
+        const uint32_t ai = (uint32_t)(batch * kMbThreads + tid);
+        const int64_t q = (int64_t)randint_mod(draw_u64(a.key, ai, 0, t32, TAG_RANDOM_EDGE),
+                                               (uint32_t)E2);
+        const int32_t u = a.g.esrc[q], v = a.g.cols[q];
+        const uint32_t lo = (uint32_t)min(u, v), hi = (uint32_t)max(u, v);
+        const bool valid = !pair_held_out(a.g, lo, hi);
+        int ts = -1;
+        if (valid) {
+          ts = table_insert64(keys, cmask, (unsigned long long)edge_key(lo, hi));
+          atomicMin(&att[ts], (int)ai);
+        }
+        __threadfence();
+        __syncthreads();
+        const int win = valid && load_agent(&att[ts]) == (int)ai;
+        int total;
+        const int pos = block_scan(win, s_wave, &total);
+        if (win && c + pos < m) {
+          mb_edges[2 * (c + pos)] = (int32_t)lo;
+          mb_edges[2 * (c + pos) + 1] = (int32_t)hi;
+          mb_y[c + pos] = 1;
+        }
+        c = min(m, c + (int64_t)total);
+        if (c >= m) break;
+      }
+    }
+    const double dN = (double)a.g.N;
+    scale = (dN - 1.0) * dN / 2.0 / (double)(c > 0 ? c : 1);  // network.cc:262
   } else if (flag == 1) {
     int* s_deg = s_row;
     const int nsrc = (int)min<int64_t>(a.max_source, kMaxSources);

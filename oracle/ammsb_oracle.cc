@@ -23,8 +23,10 @@
 //   orc_phi_grads / orc_beta_grads   the closed-form gradients of update_phi
 //                    (distr.cc:1165-1197) and update_beta (:202-240)
 //   orc_sample_minibatch   Network::stratified_random_node_sampling{,_links,
-//                    _nonlinks}, network.cc:274-421, with the counter-based
-//                    draw addressing of this project (see below)
+//                    _nonlinks}, network.cc:274-421, and random_edge_sampling
+//                    with sample_random_edges / sample_full_training_set,
+//                    network.cc:237-268, 448-500, with the counter-based draw
+//                    addressing of this project (see below)
 //   orc_sample_neighbors   sample_neighbor_nodes, mcmc_sampler_stochastic.h:87-108
 // Float is a template parameter (config.h.in:14-20); the code keeps the
 // reference's operation order and accumulation types (np::sum is a
@@ -89,7 +91,8 @@ enum : uint32_t {
   T_PHI_NOISE = 5,
   T_THETA_NOISE = 6,
   T_INIT_PHI = 7,
-  T_INIT_THETA = 8
+  T_INIT_THETA = 8,
+  T_RANDOM_EDGE = 11
 };
 
 struct Stream {
@@ -134,6 +137,7 @@ struct Net {
   std::unordered_map<uint64_t, uint8_t> heldtest;  // bit0 held-out, bit1 test
   std::vector<int32_t> ho_edges;                   // held-out in given order
   std::vector<uint8_t> ho_labels;
+  int64_t n_test = 0;                              // test entries (links + non-links)
 
   static uint64_t key(int32_t a, int32_t b) {
     if (a > b) std::swap(a, b);
@@ -144,6 +148,11 @@ struct Net {
   }
   bool in_heldtest(int32_t a, int32_t b) const { return heldtest.count(key(a, b)) != 0; }
   int64_t degree(int32_t a) const { return row_ptr[a + 1] - row_ptr[a]; }
+  // row holding directed entry q of cols (np::find_le on the cumulative
+  // degrees, network.cc:459-468)
+  int32_t row_of(int64_t q) const {
+    return (int32_t)(std::upper_bound(row_ptr.begin(), row_ptr.end(), q) - row_ptr.begin() - 1);
+  }
 };
 
 // ---- samplers (sequential definitions) -------------------------------------
@@ -158,6 +167,14 @@ struct Net {
 //               that is not held-out/test and not already in S, in CSR
 //               order; repeat rounds while S is empty (network.cc:361-383);
 //               scale = N if max_source == 1 else E/|S| (network.cc:385-390)
+// Random edge (network.cc:237-268): if m >= E - |held-out| - |test| (entry
+//   counts, links and non-links, as the reference compares), every training
+//   link (a<b, not held-out/test) in CSR order (sample_full_training_set);
+//   else attempt j: q = u64(j,0,t,RANDOM_EDGE) % 2E, the directed entry q of
+//   the CSR (row a, column b; sample_random_edges draws over both
+//   directions, network.cc:448-500) gives (min, max), accepted iff not
+//   held-out/test and not accepted before; until m accepted.  All y = 1;
+//   scale = N(N-1)/2 / |S| (network.cc:262).
 int sample_minibatch(const Net& g, const Stream& rs, int strategy, int64_t m, int64_t max_source,
                      int64_t t, int* flag_out, double* scale_out, std::vector<int32_t>* edges,
                      std::vector<uint8_t>* ys) {
@@ -170,11 +187,41 @@ int sample_minibatch(const Net& g, const Stream& rs, int strategy, int64_t m, in
     flag = 1;
   else if (strategy == 2)
     flag = 0;
+  else if (strategy == 3)
+    flag = 2;
   else
     return -1;
   edges->clear();
   ys->clear();
-  if (flag == 0) {
+  if (flag == 2) {
+    const int64_t held_test = (int64_t)g.ho_labels.size() + g.n_test;
+    if (m >= g.E - held_test) {
+      for (int32_t a = 0; a < g.N; ++a)
+        for (int64_t q = g.row_ptr[a]; q < g.row_ptr[a + 1]; ++q) {
+          const int32_t b = g.cols[q];
+          if (a < b && !g.in_heldtest(a, b)) {
+            edges->push_back(a);
+            edges->push_back(b);
+            ys->push_back(1);
+          }
+        }
+    } else {
+      std::unordered_set<uint64_t> taken;
+      const uint64_t E2 = 2 * (uint64_t)g.E;
+      for (uint64_t j = 0; (int64_t)taken.size() < m; ++j) {
+        if (j > (uint64_t)(1u << 31)) return -2;
+        const int64_t q = (int64_t)(rs.u64((uint32_t)j, 0, t32, T_RANDOM_EDGE) % E2);
+        const int32_t a = g.row_of(q), b = g.cols[q];
+        if (g.in_heldtest(a, b)) continue;
+        if (!taken.insert(Net::key(a, b)).second) continue;
+        edges->push_back(std::min(a, b));
+        edges->push_back(std::max(a, b));
+        ys->push_back(1);
+      }
+    }
+    const double dN = (double)g.N;
+    *scale_out = (dN - 1.0) * dN / 2.0 / (double)ys->size();
+  } else if (flag == 0) {
     const int32_t node = (int32_t)(rs.u64(0, 0, t32, T_SOURCE) % N);
     std::unordered_set<int32_t> taken;
     for (uint64_t j = 0; (int64_t)taken.size() < m; ++j) {
@@ -595,6 +642,7 @@ void* orc_create(int32_t N, int64_t E, const int64_t* row_ptr, const int32_t* co
     g.ho_labels.push_back(ho_labels[i]);
   }
   for (int64_t i = 0; i < T; ++i) g.heldtest[Net::key(te_edges[2 * i], te_edges[2 * i + 1])] |= 2;
+  g.n_test = T;
   (void)te_labels;
   h->prec = precision;
   const Hyper hp = hyper_from(hyper);

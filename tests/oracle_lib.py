@@ -135,7 +135,10 @@ class Oracle:
         return lib.orc_eps_t(_p(self.hyper, c_dbl), step, self.prec)
 
     def sample_minibatch(self, strategy, m, max_source, seed, step, cap=None):
-        cap = cap or (m + int(self.graph.degree().max()) + 8)
+        # random-edge: the full training set (m >= E - |held-out| - |test|)
+        # can hold more than m links
+        cap = cap or (max(m, self.graph.num_edges if strategy == 3 else 0)
+                      + int(self.graph.degree().max()) + 8)
         edges = np.zeros((cap, 2), dtype=np.int32)
         y = np.zeros(cap, dtype=np.uint8)
         flag, scale, cnt = c_i32(), c_dbl(), c_i64()

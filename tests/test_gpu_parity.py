@@ -236,3 +236,45 @@ def test_error_paths():
         M.MCMCSamplerStochastic(M.Options(K=K, num_node_sample=N), g, s)
     smp.step(3)
     assert smp.info["step_count"] == 4
+
+
+@pytest.mark.parametrize("N,m,deg", [(20000, 1500, 20.0), (300, 299, 2.0)])
+def test_random_edge_sampler_bit_exact(N, m, deg):
+    """random-edge minibatches (network.cc:237-268): the device draws equal
+    the oracle's sequential definition; m >= E - |held-out| - |test| takes
+    the whole training set."""
+    print(f"random-edge N={N} m={m}", flush=True)
+    K, n = 16, 20
+    g, s, smp, orc = make_pair("community", N, K, n=n, m=m, strategy="random-edge", seed=31,
+                               deg=deg)
+    mm = smp.info["mini_batch_size"]
+    full = mm >= g.num_edges - len(s.held_out_labels) - len(s.test_labels)
+    assert full == (deg < 5)  # the second case takes the whole training set
+    for step in range(1, 6):
+        d = smp.debug_sample(step)
+        r = orc.sample_minibatch(3, mm, 1, 31, step)
+        assert d["flag"] == r["flag"] == 2
+        assert np.array_equal(d["edges"], r["edges"]), step
+        assert d["scale"] == pytest.approx(r["scale"], rel=1e-15)
+        assert np.array_equal(np.sort(d["nodes"]), np.unique(r["edges"]))
+        for q in range(0, len(d["nodes"]), max(1, len(d["nodes"]) // 10)):
+            v = int(d["nodes"][q])
+            assert np.array_equal(d["nbrs"][q], orc.sample_neighbors(v, n + 1, 31, step))
+
+
+def test_random_edge_iteration_fp64():
+    """One full FP64 iteration with random-edge minibatches against the
+    oracle from the same state and seed (1e-12)."""
+    print("random-edge iteration", flush=True)
+    N, K = 6000, 32
+    g, s, smp, orc = make_pair("community", N, K, n=16, m=700, strategy="random-edge", seed=8)
+    pi, theta = _state(N, K, 4)
+    smp.set_state(pi, theta)
+    orc.set_state(pi, theta)
+    smp.step(1)
+    orc.run(1, 3, 700, 1, 10 ** 9, 8)
+    gpi, gth, gbe = smp.get_state()
+    rpi, rth, rbe = orc.get_state()
+    for gv, rv in ((gpi, rpi), (gth, rth), (gbe, rbe)):
+        ok, err = _close(gv, rv, TOL64)
+        assert ok, err

@@ -339,6 +339,68 @@ def py_neighbors(g, s, node, nnb, seed, t):
     return np.array(out, dtype=np.uint32).astype(np.int32)
 
 
+def py_random_edge(g, s, m, seed, t):
+    """random_edge_sampling (network.cc:237-268): uniform directed entries of
+    the CSR, distinct training links, in attempt order; the full training set
+    in CSR order when m >= E - |held-out| - |test|."""
+    ht = _heldtest(s)
+    N = g.N
+    if m >= g.num_edges - len(s.held_out_labels) - len(s.test_labels):
+        out = [(a, int(b)) for a in range(N) for b in g.cols[g.row_ptr[a]:g.row_ptr[a + 1]]
+               if a < b and (a, int(b)) not in ht]
+    else:
+        out, taken, j = [], set(), 0
+        while len(out) < m:
+            q = _u64((j, 0, t, 11), seed) % (2 * g.num_edges)
+            j += 1
+            a = int(np.searchsorted(g.row_ptr, q, side="right") - 1)
+            b = int(g.cols[q])
+            e = (min(a, b), max(a, b))
+            if e in ht or e in taken:
+                continue
+            taken.add(e)
+            out.append(e)
+    return out, (N - 1) * N / 2.0 / len(out)
+
+
+@pytest.mark.parametrize("seed", [1, 99])
+def test_random_edge_matches_python_restatement(seed):
+    g = tiny_graph(60, 5)
+    s = tiny_split(g)
+    o = Oracle(g, s, 3, num_node_sample=6)
+    n_train = g.num_edges - len(s.held_out_labels) - len(s.test_labels)
+    for t in range(1, 10):
+        for m in (10, n_train // 2, n_train + 5):
+            d = o.sample_minibatch(3, m, 1, seed, t)
+            ref, scale = py_random_edge(g, s, m, seed, t)
+            assert d["flag"] == 2
+            assert [tuple(e) for e in d["edges"]] == ref
+            assert np.all(d["y"] == 1)
+            assert d["scale"] == pytest.approx(scale, rel=1e-15)
+
+
+def test_random_edge_invariants_and_uniformity():
+    """Distinct training links only; every training link equally likely."""
+    g = tiny_graph(30, 2)
+    s = tiny_split(g)
+    ht = _heldtest(s)
+    o = Oracle(g, s, 3, num_node_sample=6)
+    train = [(a, int(b)) for a in range(g.N) for b in g.cols[g.row_ptr[a]:g.row_ptr[a + 1]]
+             if a < b and (a, int(b)) not in ht]
+    counts = dict.fromkeys(train, 0)
+    for t in range(1, 2001):
+        d = o.sample_minibatch(3, 3, 1, 7, t)
+        e = [tuple(x) for x in d["edges"]]
+        assert len(e) == len(set(e)) == 3
+        for x in e:
+            assert g.linked(*x) and x not in ht
+            counts[x] += 1
+    c = np.array(list(counts.values()), dtype=float)
+    exp = c.sum() / len(c)
+    chi2 = ((c - exp) ** 2 / exp).sum()
+    assert chi2 < len(c) + 6 * np.sqrt(2 * len(c))
+
+
 @pytest.mark.parametrize("seed", [1, 99])
 def test_samplers_match_python_restatement(seed):
     g = tiny_graph(60, 5)
