@@ -180,6 +180,17 @@ int ammsb_split_generate(const ammsb_graph *g, double held_out_ratio,
                          uint8_t **held_out_labels, int64_t *held_out_count,
                          int32_t **test_edges, uint8_t **test_labels,
                          int64_t *test_count);
+/* Edge-list reader of the reference's "relativity" input class
+ * (preprocess/relativity.cc:20-193, data_factory.cc:7-45): skips the first 4
+ * (header) lines, then one pair of integer ids per line; plain or gzip.
+ * contiguous != 0: ids as given, N = max id + 1 (--mcmc.input.contiguous);
+ * else ids renumbered 0..N-1 in order of first appearance.  Self-links and
+ * repeated pairs are dropped and counted (self_links, duplicates may be
+ * NULL).  *row_ptr (N+1) and *cols (2E, ascending rows) are allocated by the
+ * library; release with ammsb_free(). */
+int ammsb_graph_load_edge_list(const char *path, int32_t contiguous, int32_t *N,
+                               int64_t *num_edges, int64_t **row_ptr, int32_t **cols,
+                               int64_t *self_links, int64_t *duplicates);
 void ammsb_free(void *p);
 /* One Philox4x32-10 block of the library's counter-based stream:
  * out = philox(ctr[4], key[2]) (Salmon et al., SC'11).  The stream replaces

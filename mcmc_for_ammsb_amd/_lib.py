@@ -143,6 +143,9 @@ def _load():
                                          ctypes.POINTER(P_i32), ctypes.POINTER(P_u8), P_i64,
                                          ctypes.POINTER(P_i32), ctypes.POINTER(P_u8), P_i64]),
         "ammsb_free": (None, [c_vp]),
+        "ammsb_graph_load_edge_list": (c_i32, [ctypes.c_char_p, c_i32, ctypes.POINTER(c_i32),
+                                               P_i64, ctypes.POINTER(P_i64),
+                                               ctypes.POINTER(P_i32), P_i64, P_i64]),
         "ammsb_philox": (None, [ctypes.POINTER(ctypes.c_uint32)] * 3),
         "ammsb_create": (c_i32, [ctypes.POINTER(Options), ctypes.POINTER(Graph),
                                  ctypes.POINTER(Split), ctypes.POINTER(c_vp)]),

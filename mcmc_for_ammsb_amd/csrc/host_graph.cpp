@@ -15,10 +15,15 @@
 
 #include <stdlib.h>
 #include <string.h>
+#include <zlib.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <string>
+#include <unordered_map>
 #include <unordered_set>
+#include <vector>
 
 #include "../../include/ammsb.h"
 #include "host_util.h"
@@ -369,6 +374,143 @@ extern "C" int ammsb_split_generate(const ammsb_graph* g, double ratio, uint64_t
     return AMMSB_OK;
   } catch (const std::bad_alloc&) {
     set_error("ammsb_split_generate: out of memory");
+    return AMMSB_ERR_OOM;
+  }
+}
+
+// SNAP edge list, the reference's Relativity reader (relativity.cc:20-193):
+// the first 4 lines are the header; every further line holds two integer
+// ids.  contiguous: ids are used as they are and N = max id + 1
+// (relativity.cc:64-104); else ids are renumbered 0..N-1 in order of first
+// appearance (the reference keeps its unordered_set's order,
+// relativity.cc:106-150, which no other build reproduces).  Self-links and
+// repeated pairs are dropped (relativity.cc:152-172).  gzip files are read
+// through zlib (the reference's .gz inputs, relativity.cc:24-46).
+extern "C" int ammsb_graph_load_edge_list(const char* path, int32_t contiguous, int32_t* N_out,
+                                          int64_t* num_edges, int64_t** row_ptr_out,
+                                          int32_t** cols_out, int64_t* self_links,
+                                          int64_t* duplicates) {
+  if (!path || !N_out || !num_edges || !row_ptr_out || !cols_out) {
+    set_error("ammsb_graph_load_edge_list: null argument");
+    return AMMSB_ERR_ARG;
+  }
+  *row_ptr_out = nullptr;
+  *cols_out = nullptr;
+  gzFile f = gzopen(path, "rb");
+  if (!f) {
+    set_error(std::string("Cannot open ") + path);
+    return AMMSB_ERR_ARG;
+  }
+  try {
+    std::vector<int64_t> raw;  // pairs of ids as read
+    std::string line;
+    char buf[4096];
+    int64_t lineno = 0;
+    bool more = true;
+    while (more) {
+      line.clear();
+      while (true) {  // one line of any length
+        if (!gzgets(f, buf, (int)sizeof buf)) {
+          more = false;
+          break;
+        }
+        line += buf;
+        if (!line.empty() && line.back() == '\n') break;
+      }
+      if (!more && line.empty()) break;
+      ++lineno;
+      if (lineno <= 4) continue;  // header (relativity.cc:51-55)
+      const char* p = line.c_str();
+      char* end = nullptr;
+      const long long a = strtoll(p, &end, 10);
+      if (end == p) {
+        gzclose(f);
+        set_error("Fail to parse int (line " + std::to_string(lineno) + " of " + path + ")");
+        return AMMSB_ERR_ARG;
+      }
+      p = end;
+      const long long b = strtoll(p, &end, 10);
+      if (end == p) {
+        gzclose(f);
+        set_error("Fail to parse int (line " + std::to_string(lineno) + " of " + path + ")");
+        return AMMSB_ERR_ARG;
+      }
+      raw.push_back(a);
+      raw.push_back(b);
+    }
+    gzclose(f);
+    const int64_t npairs = (int64_t)raw.size() / 2;
+    int64_t N = 0;
+    if (contiguous) {
+      long long mx = -1;
+      for (int64_t i = 0; i < 2 * npairs; ++i) {
+        if (raw[i] < 0) {
+          set_error("ammsb_graph_load_edge_list: negative id in contiguous input");
+          return AMMSB_ERR_ARG;
+        }
+        mx = std::max<long long>(mx, raw[i]);
+      }
+      N = mx + 1;
+    } else {
+      std::unordered_map<int64_t, int64_t> ids;
+      ids.reserve((size_t)(2 * npairs));
+      for (int64_t i = 0; i < 2 * npairs; ++i) {
+        auto it = ids.find(raw[i]);
+        if (it == ids.end()) it = ids.emplace(raw[i], (int64_t)ids.size()).first;
+        raw[i] = it->second;
+      }
+      N = (int64_t)ids.size();
+    }
+    if (N < 1 || N > INT32_MAX) {
+      set_error("ammsb_graph_load_edge_list: node count out of range");
+      return AMMSB_ERR_ARG;
+    }
+    std::vector<std::pair<int32_t, int32_t>> e;
+    e.reserve((size_t)npairs);
+    int64_t self = 0;
+    for (int64_t i = 0; i < npairs; ++i) {
+      const int32_t a = (int32_t)raw[2 * i], b = (int32_t)raw[2 * i + 1];
+      if (a == b) {
+        ++self;
+        continue;
+      }
+      e.emplace_back(std::min(a, b), std::max(a, b));
+    }
+    std::vector<int64_t>().swap(raw);
+    std::sort(e.begin(), e.end());
+    const int64_t before = (int64_t)e.size();
+    e.erase(std::unique(e.begin(), e.end()), e.end());
+    const int64_t E = (int64_t)e.size();
+    int64_t* rp = (int64_t*)malloc(sizeof(int64_t) * (size_t)(N + 1));
+    int32_t* cols = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(1, 2 * E));
+    if (!rp || !cols) {
+      free(rp);
+      free(cols);
+      set_error("ammsb_graph_load_edge_list: out of memory");
+      return AMMSB_ERR_OOM;
+    }
+    std::fill(rp, rp + N + 1, 0);
+    for (const auto& x : e) {
+      rp[x.first + 1]++;
+      rp[x.second + 1]++;
+    }
+    for (int64_t i = 0; i < N; ++i) rp[i + 1] += rp[i];
+    std::vector<int64_t> fill(rp, rp + N);
+    for (const auto& x : e) {
+      cols[fill[x.first]++] = x.second;
+      cols[fill[x.second]++] = x.first;
+    }
+    for (int64_t i = 0; i < N; ++i) std::sort(cols + rp[i], cols + rp[i + 1]);
+    *N_out = (int32_t)N;
+    *num_edges = E;
+    *row_ptr_out = rp;
+    *cols_out = cols;
+    if (self_links) *self_links = self;
+    if (duplicates) *duplicates = before - E;
+    return AMMSB_OK;
+  } catch (const std::bad_alloc&) {
+    gzclose(f);
+    set_error("ammsb_graph_load_edge_list: out of memory");
     return AMMSB_ERR_OOM;
   }
 }

@@ -62,23 +62,64 @@ class Graph:
         return cls(N, row_ptr, dst.astype(np.int32))
 
     @classmethod
-    def from_edge_list_file(cls, path):
-        """SNAP-style edge list ('#' comments, two ids per line), node ids
-        remapped densely in order of first appearance, as the reference's
-        Relativity parser does (relativity.cc:20-193)."""
-        ids = {}
-        pairs = []
-        with open(path) as f:
-            for line in f:
-                line = line.strip()
-                if not line or line.startswith("#"):
-                    continue
-                a, b = line.split()[:2]
-                for x in (a, b):
-                    if x not in ids:
-                        ids[x] = len(ids)
-                pairs.append((ids[a], ids[b]))
-        return cls.from_edges(len(ids), np.array(pairs, dtype=np.int64).reshape(-1, 2))
+    def from_edge_list_file(cls, path, contiguous=False):
+        """SNAP edge list read by the native loader with the reference's
+        "relativity" rules (relativity.cc:20-193): the first 4 lines are the
+        header, then two integer ids per line, plain or gzip; contiguous ids
+        are kept (N = max id + 1), others are renumbered in order of first
+        appearance; self-links and repeated pairs are dropped."""
+        N = ctypes.c_int32()
+        E = ctypes.c_int64()
+        rp = ctypes.POINTER(ctypes.c_int64)()
+        cl = ctypes.POINTER(ctypes.c_int32)()
+        self_links, dups = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(_lib.lib.ammsb_graph_load_edge_list(
+            str(path).encode(), 1 if contiguous else 0, ctypes.byref(N), ctypes.byref(E),
+            ctypes.byref(rp), ctypes.byref(cl), ctypes.byref(self_links), ctypes.byref(dups)))
+        g = cls(int(N.value), _take(rp, N.value + 1, np.int64), _take(cl, 2 * E.value, np.int32))
+        g.self_links, g.duplicates = int(self_links.value), int(dups.value)
+        return g
+
+    @classmethod
+    def from_netscience_xml(cls, path):
+        """The reference's "netscience" input class (netscience.cc:28-134):
+        N = the number of <node> elements of the first nodeclass, one edge per
+        <link source= target=> of the first network."""
+        import xml.etree.ElementTree as ET  # parses text only
+        root = ET.parse(str(path)).getroot()
+        if root.tag != "DynamicNetwork":
+            raise ValueError("Cannot get 'DynamicNetwork'")
+        meta = root.find("MetaNetwork")
+        if meta is None:
+            raise ValueError("Cannot get 'MetaNetwork'")
+        nodeclass = meta.find("nodes/nodeclass")
+        network = meta.find("networks/network")
+        if nodeclass is None or network is None:
+            raise ValueError("Cannot get 'nodes/nodeclass' or 'networks/network'")
+        ids = [int(n.get("id")) for n in nodeclass.findall("node")]
+        N = len(set(ids))
+        links = [(int(l.get("target")), int(l.get("source"))) for l in network.findall("link")]
+        e = np.array(links, dtype=np.int64).reshape(-1, 2)
+        if len(e) and (e.min() < 0 or e.max() >= N):
+            raise ValueError("link id outside the node range")
+        return cls.from_edges(N, e)
+
+    @classmethod
+    def from_options(cls, options):
+        """The reference's DataFactory (data_factory.cc:7-60): input_class
+        relativity / rc / rz / rcz (contiguous and compressed switches) or
+        netscience, reading options.input_filename."""
+        klass = options.input_class
+        contiguous = options.input_contiguous or klass in ("rc", "rcz")
+        if klass in ("relativity", "rc", "rz", "rcz"):
+            return cls.from_edge_list_file(options.input_filename, contiguous=contiguous)
+        if klass == "netscience":
+            return cls.from_netscience_xml(options.input_filename or "datasets/netscience.xml")
+        if klass in ("sparsehash", "gz", "preprocessed"):
+            raise NotImplementedError(
+                f"input class {klass!r} is google sparsehash's serialized table format, which "
+                "only sparsehash itself reads; convert the graph to an edge list")
+        raise ValueError(f'Unknown dataset name "{klass}"')
 
 
 @dataclasses.dataclass

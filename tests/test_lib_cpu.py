@@ -107,12 +107,48 @@ def test_split_semantics():
 
 
 def test_graph_from_edges_and_file(tmp_path):
+    """relativity.cc:20-193: 4 header lines, dense renumbering in order of
+    first appearance, self-links and repeated pairs dropped, gzip input,
+    contiguous ids (N = max + 1)."""
+    import gzip
+    body = "10 20\n20 10\n10 30\n30 30\n40 10\n"
+    header = "# Directed graph\n# toy\n# Nodes: 4 Edges: 4\n# FromNodeId\tToNodeId\n"
     f = tmp_path / "g.txt"
-    f.write_text("# comment\n10 20\n20 10\n10 30\n30 30\n40 10\n")
+    f.write_text(header + body)
     g = M.Graph.from_edge_list_file(str(f))
-    assert g.N == 4 and g.num_edges == 3
+    assert g.N == 4 and g.num_edges == 3 and g.self_links == 1 and g.duplicates == 1
     _check_csr(g)
     assert g.linked(0, 1) and g.linked(0, 2) and g.linked(0, 3) and not g.linked(1, 2)
+    fz = tmp_path / "g.txt.gz"
+    with gzip.open(fz, "wt") as h:
+        h.write(header + body)
+    gz = M.Graph.from_edge_list_file(str(fz))
+    assert np.array_equal(gz.row_ptr, g.row_ptr) and np.array_equal(gz.cols, g.cols)
+    gc = M.Graph.from_edge_list_file(str(f), contiguous=True)
+    assert gc.N == 41 and gc.num_edges == 3 and gc.linked(10, 20) and gc.linked(10, 40)
+    bad = tmp_path / "bad.txt"
+    bad.write_text(header + "1 2\nx y\n")
+    with pytest.raises(M.AmmsbError, match="Fail to parse"):
+        M.Graph.from_edge_list_file(str(bad))
+    with pytest.raises(M.AmmsbError, match="Cannot open"):
+        M.Graph.from_edge_list_file(str(tmp_path / "missing.txt"))
+
+
+def test_graph_from_options_and_netscience(tmp_path):
+    xml = tmp_path / "ns.xml"
+    xml.write_text('<DynamicNetwork><MetaNetwork><nodes><nodeclass id="agent" type="agent">'
+                   '<node id="0" title="A"/><node id="1" title="B"/><node id="2" title="C"/>'
+                   '</nodeclass></nodes><networks><network id="agent x agent" isDirected="false">'
+                   '<link source="1" target="0" value="2.5"/><link source="2" target="1"/>'
+                   '</network></networks></MetaNetwork></DynamicNetwork>')
+    g = M.Graph.from_options(Options.parse(["--mcmc.input.class", "netscience", "-f", str(xml)]))
+    assert g.N == 3 and g.num_edges == 2 and g.linked(0, 1) and g.linked(1, 2)
+    f = tmp_path / "e.txt"
+    f.write_text("#\n#\n#\n#\n0 1\n1 2\n")
+    g = M.Graph.from_options(Options.parse(["-c", "rc", "-f", str(f)]))
+    assert g.N == 3 and g.num_edges == 2
+    with pytest.raises(NotImplementedError):
+        M.Graph.from_options(Options.parse(["-c", "preprocessed", "-f", str(tmp_path)]))
 
 
 def test_sampler_requires_graph_type():
@@ -156,3 +192,31 @@ def test_batch_create_rejects_unsupported_chains():
                      M.Options(K=16, num_node_sample=10, precision="fp64")], [g, g], [s, s])
     with pytest.raises(ValueError):
         M.MCMCBatch([M.Options(K=16)], [g, g])
+
+
+REF_DATASETS = "/root/reference/datasets"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_DATASETS), reason="reference datasets not present")
+@pytest.mark.parametrize("name", ["CA-GrQc-tiny.txt", "network.txt", "netscience.txt",
+                                  "CA-HepTh.txt"])
+def test_reference_datasets_load(name):
+    """The reference's own SNAP files through the native reader against an
+    independent Python parse of the same rules (4 header lines, undirected
+    pairs, no self-links)."""
+    path = os.path.join(REF_DATASETS, name)
+    g = M.Graph.from_edge_list_file(path)
+    ids, pairs = {}, set()
+    with open(path) as f:
+        for i, line in enumerate(f):
+            if i < 4:
+                continue
+            a, b = (int(x) for x in line.split()[:2])
+            for x in (a, b):
+                ids.setdefault(x, len(ids))
+            if a != b:
+                pairs.add((min(ids[a], ids[b]), max(ids[a], ids[b])))
+    assert g.N == len(ids) and g.num_edges == len(pairs)
+    _check_csr(g)
+    for a, b in list(pairs)[:200]:
+        assert g.linked(a, b) and g.linked(b, a)
