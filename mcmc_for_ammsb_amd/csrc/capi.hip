@@ -10,6 +10,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <deque>
@@ -613,13 +614,26 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
   ALLOC_TRY(c->d_ho_labels, (size_t)hc);
   ALLOC_TRY(c->d_R, (size_t)hc);
   if (hc) {
+    // Held-out pairs sorted by (a, b): the pairs of one node a are adjacent,
+    // so the perplexity pass reads row a once for all of them (concurrent
+    // misses to one line merge in L2).  The perplexity is a sum over pairs,
+    // so the order changes nothing but the summation order.
+    std::vector<int64_t> order(hc);
+    for (int64_t i = 0; i < hc; ++i) order[i] = i;
+    auto lo = [&](int64_t i) { return std::min(ho[2 * i], ho[2 * i + 1]); };
+    auto hi = [&](int64_t i) { return std::max(ho[2 * i], ho[2 * i + 1]); };
+    std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+      return lo(x) != lo(y) ? lo(x) < lo(y) : hi(x) < hi(y);
+    });
     std::vector<int32_t> hoe(2 * hc);
-    for (int64_t i = 0; i < hc; ++i) {
-      hoe[2 * i] = std::min(ho[2 * i], ho[2 * i + 1]);
-      hoe[2 * i + 1] = std::max(ho[2 * i], ho[2 * i + 1]);
+    std::vector<uint8_t> hl(hc);
+    for (int64_t j = 0; j < hc; ++j) {
+      hoe[2 * j] = lo(order[j]);
+      hoe[2 * j + 1] = hi(order[j]);
+      hl[j] = hol[order[j]];
     }
     HIP_TRY(hipMemcpy(c->d_ho_edges, hoe.data(), 8 * hc, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_ho_labels, hol, hc, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_ho_labels, hl.data(), hc, hipMemcpyHostToDevice));
   }
   if (own_split) {
     free(ho);

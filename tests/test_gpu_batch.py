@@ -116,13 +116,15 @@ def test_graph_replay_equals_direct_launches():
     print("batch graph replay vs direct launches", flush=True)
     ga = _batch("fp32", CHAINS, graph=True)[0]
     da = _batch("fp32", CHAINS, graph=False)[0]
-    for _ in range(3):
-        ga.step(2)
-        da.step(2)
-    np.testing.assert_array_equal(ga.perplexity(), da.perplexity())
-    for i in range(len(CHAINS)):
-        for x, y in zip(ga.chain(i).get_state(), da.chain(i).get_state()):
-            np.testing.assert_array_equal(x, y)
+    # several replays with host work and synchronisation between them; every
+    # replay checked against the direct launches (crossing a refill at 16)
+    for steps in (2, 7, 9, 3):
+        ga.step(steps)
+        da.step(steps)
+        np.testing.assert_array_equal(ga.perplexity(), da.perplexity())
+        for i in range(len(CHAINS)):
+            for x, y in zip(ga.chain(i).get_state(), da.chain(i).get_state()):
+                np.testing.assert_array_equal(x, y)
     ga.close()
     da.close()
 
