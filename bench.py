@@ -422,6 +422,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--phi-waves", type=int, default=0, help="update_phi wavefronts per node")
     ap.add_argument("--phi-unroll", type=int, default=0, help="update_phi rows in flight")
+    ap.add_argument("--timer-mode", default="phi", choices=["phi", "all", "off"],
+                    help="HIP events in the timed region: update_phi only, every phase, none")
+    ap.add_argument("--phi-split", type=int, default=-1,
+                    help="1: update_phi gather + finish kernels; 0: fused step + scatter")
     ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "update_phi_traffic.json"))
     ap.add_argument("--mode", default="throughput", choices=["throughput", "ttt", "diff", "batch"])
     ap.add_argument("--chains", type=int, default=64)
@@ -451,13 +455,18 @@ def main():
         cur = smp.get_tuning()
         smp.set_tuning(phi_waves=args.phi_waves or cur["phi_waves"],
                        phi_unroll=args.phi_unroll or cur["phi_unroll"])
+    if args.phi_split >= 0:
+        smp.set_tuning(phi_split=args.phi_split)
     inf = smp.info
     smp.init(opts.random_seed)
     K, nnb = inf["K"], inf["num_node_sample"] + 1
     fsize = 8 if opts.precision == "fp64" else 4
 
     smp.run(max_steps=args.warmup, print_log=False)
-    smp.enable_timers(True)
+    # timed region: HIP events around update_phi only (for the roofline);
+    # the per-phase breakdown comes from a separate pass below, because an
+    # event pair around every phase adds gaps to the stream
+    smp.enable_timers(2 if args.timer_mode == "phi" else (1 if args.timer_mode == "all" else 0))
     t_before = smp.timers()
     n0, e0 = smp.totals()
     t0 = time.perf_counter()
@@ -471,9 +480,21 @@ def main():
     value = nodes * K / elapsed
 
     ph = {k: (t_after[k][0] - t_before[k][0], t_after[k][1] - t_before[k][1]) for k in t_after}
+    bd_steps = min(args.steps, 100)
+    smp.enable_timers(True)
+    tb0 = smp.timers()
+    nb0 = smp.totals()[0]
+    smp.run(max_steps=bd_steps, print_log=False)
+    nb1 = smp.totals()[0]
+    tb1 = smp.timers()
+    smp.enable_timers(False)
+    ph_all = {k: (tb1[k][0] - tb0[k][0], tb1[k][1] - tb0[k][1]) for k in tb1}
+    phi_nodes = nodes
+    if args.timer_mode == "off":
+        ph, phi_nodes = ph_all, nb1 - nb0  # roofline from the breakdown pass
     phi_s, phi_n = ph["update_phi"]
     avg_launch = phi_s / max(phi_n, 1)
-    nodes_per_launch = nodes / max(phi_n, 1)
+    nodes_per_launch = phi_nodes / max(phi_n, 1)
     # algorithmic bytes of one (node, k) update: n_nb neighbour rows + own row
     # read + own row written; per node the n_nb neighbour ids and phi_sum
     bytes_per_node = (nnb + 2) * K * fsize + nnb * 4 + 2 * fsize
@@ -491,8 +512,9 @@ def main():
 
     probe_s, probe_bytes = smp.probe_read(5)  # contiguous read of the resident state
     log(f"contiguous read of {probe_bytes / 1e9:.2f} GB: {probe_bytes / probe_s / 1e9:.0f} GB/s")
+    log(f"phase breakdown ({bd_steps} further steps, every phase timed)")
     log("phase            total_s   ticks   per_tick_us")
-    for k, (sec, n) in ph.items():
+    for k, (sec, n) in ph_all.items():
         log(f"{k:<16} {sec:9.4f} {n:7d} {1e6 * sec / max(n, 1):12.2f}")
     log(f"checkpoints (step, perplexity): {ckpt}")
 
@@ -520,8 +542,9 @@ def main():
                      "kernel": "k_update_phi", "avg_launch_us": 1e6 * avg_launch,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "nodes_per_launch": nodes_per_launch, "launch_shape": smp.get_tuning()},
-        "update_phi_kernel_node_k_per_s": nodes * K / phi_s if phi_s > 0 else None,
-        "phase_us_per_step": {k: 1e6 * sec / args.steps for k, (sec, n) in ph.items()},
+        "update_phi_kernel_node_k_per_s": phi_nodes * K / phi_s if phi_s > 0 else None,
+        "timed_region_events": args.timer_mode,
+        "phase_us_per_step": {k: 1e6 * sec / bd_steps for k, (sec, n) in ph_all.items()},
         "perplexity_last": ckpt[-1][1] if ckpt else None,
         "stream_read_GBs": probe_bytes / probe_s / 1e9,
         "cpu_baseline": None,

@@ -252,6 +252,8 @@ int ammsb_run(ammsb_ctx *ctx, int64_t max_steps, int32_t print,
  * sample_mini_batch, nodes_in_mini_batch, sample_neighbor_nodes,
  * update_phi, update_pi, update_beta.  secs[8], ticks[8]. */
 int ammsb_timers(ammsb_ctx *ctx, double *secs, int64_t *ticks);
+/* on: 0 off, 1 every phase, 2 update_phi only (the fewest events in the
+ * iteration's stream). */
 int ammsb_enable_timers(ammsb_ctx *ctx, int32_t on);
 
 /* ---- single operators with injected draws (parity) --------------------- */

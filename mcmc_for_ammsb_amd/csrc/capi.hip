@@ -105,6 +105,7 @@ struct ammsb_ctx {
   int64_t mbtable_cap = 0;
   int32_t mbtable_in_lds = 0;
   int32_t phi_waves = kPhiWavesDefault, phi_unroll = kPhiUnrollDefault;
+  int32_t phi_split = 0;  // 1: gather + finish kernels instead of fused step + scatter
   // K-chunked log-likelihood launch shapes [0] forward, [1] adjoint
   int32_t ll_pairs[2] = {0, 0}, ll_prefetch[2] = {0, 0};
   double* d_ll_scratch = nullptr;   // differentiation: per-source values / dbeta partials
@@ -122,6 +123,7 @@ struct ammsb_ctx {
   bool state_ready = false;
   // timers
   bool timers_on = false;
+  uint32_t timer_mask = 0;  // phases timed (bit = Phase)
   double t_secs[PH_COUNT] = {0};
   int64_t t_ticks[PH_COUNT] = {0};
   std::vector<PendingEvent> pending;
@@ -211,7 +213,7 @@ struct PhaseScope {
   int phase;
   hipEvent_t a = nullptr;
   PhaseScope(ammsb_ctx* c_, int ph) : c(c_), phase(ph) {
-    if (c->timers_on) {
+    if (c->timers_on && (c->timer_mask >> ph & 1u)) {
       a = take_event(c);
       if (a) (void)hipEventRecord(a, c->stream);
     }
@@ -360,14 +362,19 @@ int enqueue_iteration(ammsb_ctx* c) {
     c->ring_base = c->step_count;
     c->ring_valid = true;
   }
+  const bool split = c->phi_split && c->phi_waves == kPhiWavesDefault &&
+                     c->phi_unroll == kPhiUnrollDefault;
   {
     PhaseScope ps(c, PH_UPDATE_PHI);
     HIP_TRY(launch_update_phi(phi_args(c), c->prec, (int32_t)c->node_cap, c->stream, c->phi_waves,
-                              c->phi_unroll));
+                              c->phi_unroll, split));
   }
   {
     PhaseScope ps(c, PH_UPDATE_PI);
-    HIP_TRY(launch_scatter(scatter_args(c), c->prec, (int32_t)c->node_cap, c->stream));
+    if (split)
+      HIP_TRY(launch_phi_finish(phi_args(c), c->prec, (int32_t)c->node_cap, c->stream));
+    else
+      HIP_TRY(launch_scatter(scatter_args(c), c->prec, (int32_t)c->node_cap, c->stream));
   }
   {
     PhaseScope ps(c, PH_UPDATE_BETA);
@@ -1022,6 +1029,8 @@ int ammsb_enable_timers(ammsb_ctx* c, int32_t on) {
   if (!c) return AMMSB_ERR_ARG;
   if (!on) flush_timers(c);
   c->timers_on = on != 0;
+  // 1: every phase; 2: update_phi only (two events per iteration)
+  c->timer_mask = on == 2 ? (1u << PH_UPDATE_PHI) : 0xFFFFFFFFu;
   return AMMSB_OK;
 }
 
@@ -1090,6 +1099,7 @@ int ammsb_update_phi(ammsb_ctx* c, const int32_t* nodes, int64_t n_nodes, const 
     e = hipMemcpy(d_noise, noise, n_nodes * c->K * c->fsize, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = launch_label_nbrs(dev_graph(c), d_nodes, (int32_t)n_nodes, d_nbrs, (int32_t)n_nb, c->stream);
+  bool split = false;
   if (e == hipSuccess) {
     PhiArgs pa;
     pa.pi = c->d_pi;
@@ -1109,9 +1119,12 @@ int ammsb_update_phi(ammsb_ctx* c, const int32_t* nodes, int64_t n_nodes, const 
     pa.noise = d_noise;
     pa.key = c->key;
     pa.staging = d_stage;
-    e = launch_update_phi(pa, c->prec, (int32_t)n_nodes, c->stream, c->phi_waves, c->phi_unroll);
+    split = c->phi_split && c->phi_waves == kPhiWavesDefault && c->phi_unroll == kPhiUnrollDefault;
+    e = launch_update_phi(pa, c->prec, (int32_t)n_nodes, c->stream, c->phi_waves, c->phi_unroll,
+                          split);
+    if (e == hipSuccess && split) e = launch_phi_finish(pa, c->prec, (int32_t)n_nodes, c->stream);
   }
-  if (e == hipSuccess) {
+  if (e == hipSuccess && !split) {
     ScatterArgs sc;
     sc.pi = c->d_pi;
     sc.ld = c->ld;
@@ -1253,6 +1266,11 @@ int ammsb_get_totals(ammsb_ctx* c, int64_t* nodes_updated, int64_t* minibatch_ed
 int ammsb_set_tuning(ammsb_ctx* c, const char* name, int64_t value) {
   if (!c || !name) return AMMSB_ERR_ARG;
   const std::string n(name);
+  if (n == "phi_split") {
+    if (value != 0 && value != 1) return AMMSB_ERR_ARG;
+    c->phi_split = (int32_t)value;
+    return AMMSB_OK;
+  }
   if (n == "ll_fwd_pairs" || n == "ll_fwd_prefetch" || n == "ll_bwd_pairs" ||
       n == "ll_bwd_prefetch") {
     const int d = n.compare(0, 6, "ll_bwd") == 0 ? 1 : 0;
@@ -1294,6 +1312,8 @@ int ammsb_get_tuning(ammsb_ctx* c, const char* name, int64_t* value) {
     *value = c->phi_waves;
   else if (n == "phi_unroll")
     *value = c->phi_unroll;
+  else if (n == "phi_split")
+    *value = c->phi_split;
   else if (n == "ll_fwd_pairs")
     *value = c->ll_pairs[0];
   else if (n == "ll_fwd_prefetch")

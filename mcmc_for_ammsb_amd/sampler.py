@@ -19,7 +19,7 @@ PHASES = ("outer", "perplexity", "sample_mini_batch", "nodes_in_mini_batch",
 
 
 # set in this order: each launch-shape pair is validated on its second key
-TUNING_KEYS = ("phi_waves", "phi_unroll", "ll_fwd_pairs", "ll_fwd_prefetch", "ll_bwd_pairs",
+TUNING_KEYS = ("phi_waves", "phi_unroll", "phi_split", "ll_fwd_pairs", "ll_fwd_prefetch", "ll_bwd_pairs",
                "ll_bwd_prefetch")
 
 
@@ -187,7 +187,10 @@ class MCMCSamplerStochastic:
 
     # ---- timers -----------------------------------------------------------
     def enable_timers(self, on=True):
-        _lib.check(_lib.lib.ammsb_enable_timers(self._h, 1 if on else 0))
+        """Per-phase HIP-event timers: True/1 every phase, 2 update_phi only,
+        False/0 off."""
+        mode = 2 if on == 2 else (1 if on else 0)
+        _lib.check(_lib.lib.ammsb_enable_timers(self._h, mode))
 
     def timers(self):
         secs = np.zeros(8, dtype=np.float64)

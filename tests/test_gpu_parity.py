@@ -64,12 +64,15 @@ def make_pair(kind, N, K, precision="fp64", n=32, m=None, strategy="stratified-r
     return g, s, smp, orc
 
 
+@pytest.mark.parametrize("split", [0, 1], ids=["fused", "split"])
 @pytest.mark.parametrize("kind,N,K", [("planted", 2000, 32), ("community", 20000, 256),
                                       ("community", 8000, 300), ("community", 4000, 7),
                                       ("community", 3000, 1024)])
-def test_update_phi_fp64_injected(kind, N, K):
-    print(f"update_phi fp64 {kind} N={N} K={K}", flush=True)
+def test_update_phi_fp64_injected(kind, N, K, split):
+    """split: gather kernel + k_phi_finish instead of the fused step."""
+    print(f"update_phi fp64 {kind} N={N} K={K} split={split}", flush=True)
     g, s, smp, orc = make_pair(kind, N, K, n=24)
+    smp.set_tuning(phi_split=split)
     pi, theta = _state(N, K, 7)
     smp.set_state(pi, theta)
     orc.set_state(pi, theta)
