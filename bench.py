@@ -424,6 +424,8 @@ def main():
     ap.add_argument("--phi-unroll", type=int, default=0, help="update_phi rows in flight")
     ap.add_argument("--timer-mode", default="phi", choices=["phi", "all", "off"],
                     help="HIP events in the timed region: update_phi only, every phase, none")
+    ap.add_argument("--async-ring", type=int, default=-1,
+                    help="1: samplers draw ahead on a second stream; 0: on the main stream")
     ap.add_argument("--phi-split", type=int, default=-1,
                     help="1: update_phi gather + finish kernels; 0: fused step + scatter")
     ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "update_phi_traffic.json"))
@@ -457,6 +459,8 @@ def main():
                        phi_unroll=args.phi_unroll or cur["phi_unroll"])
     if args.phi_split >= 0:
         smp.set_tuning(phi_split=args.phi_split)
+    if args.async_ring >= 0:
+        smp.set_tuning(async_ring=args.async_ring)
     inf = smp.info
     smp.init(opts.random_seed)
     K, nnb = inf["K"], inf["num_node_sample"] + 1

@@ -72,6 +72,18 @@ struct ammsb_ctx {
   ammsb_batch* owner = nullptr;     // batch that owns this chain, if any
   int32_t max_ring_slots = 16;
   int64_t ring_gen = 0;             // bumped by every single-context ring refill
+  // Sampler refills ahead on a second stream (single contexts): the ring is
+  // two halves of R slots; block k = iterations [aring_B + kR, +R) lives in
+  // half k mod 2, and block k+1 is drawn while block k computes.
+  bool async_ring = false;
+  int32_t aR = 0;
+  hipStream_t fill_stream = nullptr;
+  hipEvent_t ev_fill[2] = {nullptr, nullptr};
+  hipEvent_t ev_done[2] = {nullptr, nullptr};
+  SamplerArgs* d_sargs1 = nullptr;    // this context's sampler arguments on the device
+  NeighborArgs* d_nargs1 = nullptr;
+  bool aring_ok = false;
+  int64_t aring_B = 0, aring_hi = 0, aring_gen = -1;
   Key key{};
   DevHyper hp{};
   // device graph
@@ -341,6 +353,7 @@ ThetaArgs theta_args(const ammsb_ctx* c) {
 // device's step_count) into ring slots 0..fills-1
 int fill_ring(ammsb_ctx* c, int64_t t0, int32_t fills) {
   c->ring_gen++;
+  if (c->fill_stream) HIP_TRY(hipStreamSynchronize(c->fill_stream));  // no refill in flight
   {
     PhaseScope ps(c, PH_MINIBATCH);
     HIP_TRY(launch_minibatch(sampler_args(c, t0), fills, c->stream));
@@ -352,11 +365,69 @@ int fill_ring(ammsb_ctx* c, int64_t t0, int32_t fills) {
   return AMMSB_OK;
 }
 
+// ---- sampler refills ahead (async_ring): block k of R iterations into half
+// k mod 2 of the ring
+int aring_fill(ammsb_ctx* c, int64_t k, hipStream_t s) {
+  const int32_t R = c->aR;
+  const size_t mb_lds = c->mbtable_in_lds ? (size_t)c->mbtable_cap * sizeof(int2) : 0;
+  HIP_TRY(launch_minibatch_batch(c->d_sargs1, 1, c->aring_B + k * R, (int)(k % 2) * R, R, 2 * R,
+                                 mb_lds, s));
+  HIP_TRY(launch_neighbors_batch(c->d_nargs1, 1, (int)(k % 2) * R, R, 2 * R, (int)c->node_cap,
+                                 neighbor_lds_bytes(c->nbr_tcap), s));
+  return AMMSB_OK;
+}
+
+// Before iteration step_count: the ring set up (block 0 drawn in stream
+// order on the main stream) when it does not hold this step; at a block's
+// first iteration, wait for its draws and start drawing the next block on
+// the sampler stream, after the iterations that used its half.
+int aring_before(ammsb_ctx* c) {
+  const int64_t S = c->step_count;
+  const int32_t R = c->aR;
+  if (!c->aring_ok || c->aring_gen != c->ring_gen || S < c->aring_hi - 2 * R ||
+      S >= c->aring_hi) {
+    HIP_TRY(hipStreamSynchronize(c->fill_stream));
+    c->aring_B = S;
+    HIP_TRY(launch_set_ring(c->d_st, S, 2 * R, c->stream));
+    const int rc = aring_fill(c, 0, c->stream);
+    if (rc != AMMSB_OK) return rc;
+    HIP_TRY(hipEventRecord(c->ev_fill[0], c->stream));
+    c->aring_hi = S + R;
+    c->aring_ok = true;
+    c->aring_gen = c->ring_gen;
+    c->ring_valid = false;  // the synchronous ring bookkeeping no longer applies
+  }
+  const int64_t k = (S - c->aring_B) / R, pos = (S - c->aring_B) % R;
+  if (pos == 0) {
+    HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_fill[k % 2], 0));
+    if (c->aring_hi == c->aring_B + (k + 1) * R) {
+      HIP_TRY(hipStreamWaitEvent(c->fill_stream, c->ev_done[(k + 1) % 2], 0));
+      const int rc = aring_fill(c, k + 1, c->fill_stream);
+      if (rc != AMMSB_OK) return rc;
+      HIP_TRY(hipEventRecord(c->ev_fill[(k + 1) % 2], c->fill_stream));
+      c->aring_hi += R;
+    }
+  }
+  return AMMSB_OK;
+}
+
+int aring_after(ammsb_ctx* c) {
+  const int64_t S = c->step_count;
+  const int64_t k = (S - c->aring_B) / c->aR, pos = (S - c->aring_B) % c->aR;
+  if (pos == c->aR - 1) HIP_TRY(hipEventRecord(c->ev_done[k % 2], c->stream));
+  return AMMSB_OK;
+}
+
 // ---- one SGRLD iteration on the device (mcmc_sampler_stochastic.cc:131-174)
-// The minibatch and neighbour draws come from the ring, refilled every
-// ring.slots iterations.
+// The minibatch and neighbour draws come from the ring: drawn ahead on the
+// sampler stream (async_ring), or refilled on this stream every ring.slots
+// iterations.
 int enqueue_iteration(ammsb_ctx* c) {
-  if (!c->ring_valid || c->step_count < c->ring_base || c->step_count >= c->ring_base + c->ring.slots) {
+  if (c->async_ring) {
+    const int rc = aring_before(c);
+    if (rc != AMMSB_OK) return rc;
+  } else if (!c->ring_valid || c->step_count < c->ring_base ||
+             c->step_count >= c->ring_base + c->ring.slots) {
     int rc = fill_ring(c, c->step_count, c->ring.slots);
     if (rc != AMMSB_OK) return rc;
     c->ring_base = c->step_count;
@@ -381,6 +452,7 @@ int enqueue_iteration(ammsb_ctx* c) {
     HIP_TRY(launch_beta(beta_args(c), c->prec, c->stream));
     HIP_TRY(launch_theta(theta_args(c), c->stream));
   }
+  if (c->async_ring) return aring_after(c);
   return AMMSB_OK;
 }
 
@@ -476,6 +548,16 @@ void free_ctx(ammsb_ctx* c) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  if (c->fill_stream) {
+    (void)hipStreamSynchronize(c->fill_stream);
+    (void)hipStreamDestroy(c->fill_stream);
+  }
+  for (int h = 0; h < 2; ++h) {
+    if (c->ev_fill[h]) (void)hipEventDestroy(c->ev_fill[h]);
+    if (c->ev_done[h]) (void)hipEventDestroy(c->ev_done[h]);
+  }
+  if (c->d_sargs1) (void)hipFree(c->d_sargs1);
+  if (c->d_nargs1) (void)hipFree(c->d_nargs1);
   if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -673,8 +755,11 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
     // sample-ahead ring: up to 16 iterations, at most ~1 GiB
     const int64_t slot_bytes = c->edge_cap * 9 + c->node_cap * 4 + c->node_cap * c->nnb * 4 +
                                (int64_t)c->N * 4 + 64;
+    // single contexts: two halves of up to 16 (the samplers draw one half
+    // while the other is used)
+    const int64_t max_slots = c->owner ? c->max_ring_slots : 2 * c->max_ring_slots;
     const int64_t slots = std::max<int64_t>(
-        1, std::min<int64_t>(c->max_ring_slots, ((int64_t)1 << 30) / slot_bytes));
+        1, std::min<int64_t>(max_slots, ((int64_t)1 << 30) / slot_bytes));
     Ring& R = c->ring;
     R.slots = (int32_t)slots;
     HIP_TRY(hipMemcpy(&c->d_st->ring_slots, &R.slots, sizeof(int32_t), hipMemcpyHostToDevice));
@@ -738,6 +823,26 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
   c->ppx_blocks = (int32_t)std::max<int64_t>(
       1, std::min<int64_t>(2048, (c->H + 4 * rpw * 8 - 1) / (4 * rpw * 8)));
   ALLOC_TRY(c->d_ppx_partials, (size_t)c->ppx_blocks * 4);
+  if (!c->owner && c->ring.slots >= 2 && c->nbr_use_lds) {
+    // draw each next block of iterations on a sampler stream (see aring_before)
+    c->aR = c->ring.slots / 2;
+    c->ring.slots = 2 * c->aR;
+    HIP_TRY(hipStreamCreateWithFlags(&c->fill_stream, hipStreamNonBlocking));
+    for (int h = 0; h < 2; ++h) {
+      HIP_TRY(hipEventCreateWithFlags(&c->ev_fill[h], hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&c->ev_done[h], hipEventDisableTiming));
+    }
+    ALLOC_TRY(c->d_sargs1, 1);
+    ALLOC_TRY(c->d_nargs1, 1);
+    const SamplerArgs sa = sampler_args(c, -1);
+    const NeighborArgs na = neighbor_args(c);
+    HIP_TRY(hipMemcpy(c->d_sargs1, &sa, sizeof sa, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_nargs1, &na, sizeof na, hipMemcpyHostToDevice));
+    // off by default: measured on one MI355X, the overlap is a wash (C2 FP64
+    // 0.261 vs 0.262 ms per step, C3 FP32 0.654 vs 0.650) because the
+    // samplers take CUs from update_phi; set_tuning("async_ring", 1) enables it
+    c->async_ring = false;
+  }
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (o.verbose) {
     // learner.cc:105-120, mcmc_sampler_stochastic.cc:84-91
@@ -1271,6 +1376,18 @@ int ammsb_set_tuning(ammsb_ctx* c, const char* name, int64_t value) {
     c->phi_split = (int32_t)value;
     return AMMSB_OK;
   }
+  if (n == "async_ring") {
+    if (value != 0 && value != 1) return AMMSB_ERR_ARG;
+    if (value == 1 && !c->fill_stream) {
+      set_error("ammsb_set_tuning: this context has no sampler stream");
+      return AMMSB_ERR_ARG;
+    }
+    if (c->fill_stream) HIP_TRY(hipStreamSynchronize(c->fill_stream));
+    c->async_ring = value == 1;
+    c->aring_ok = false;
+    c->ring_valid = false;
+    return AMMSB_OK;
+  }
   if (n == "ll_fwd_pairs" || n == "ll_fwd_prefetch" || n == "ll_bwd_pairs" ||
       n == "ll_bwd_prefetch") {
     const int d = n.compare(0, 6, "ll_bwd") == 0 ? 1 : 0;
@@ -1314,6 +1431,8 @@ int ammsb_get_tuning(ammsb_ctx* c, const char* name, int64_t* value) {
     *value = c->phi_unroll;
   else if (n == "phi_split")
     *value = c->phi_split;
+  else if (n == "async_ring")
+    *value = c->async_ring ? 1 : 0;
   else if (n == "ll_fwd_pairs")
     *value = c->ll_pairs[0];
   else if (n == "ll_fwd_prefetch")

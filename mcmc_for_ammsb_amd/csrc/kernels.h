@@ -238,6 +238,8 @@ hipError_t launch_init_phi(void* pi, int64_t ld, int32_t K, int32_t N, Key key,
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 hipError_t launch_fill_f64(double* p, double v, int64_t n, hipStream_t s);
 hipError_t launch_set_state_step(DevState* st, int64_t step, hipStream_t s);
+// DevState::ring_base and ring_slots, in stream order
+hipError_t launch_set_ring(DevState* st, int64_t base, int32_t slots, hipStream_t s);
 // Which (lanes-per-row, chunks) configuration serves K; false if K too big.
 bool row_config(int32_t K, int precision, int* lpr, int* ch);
 

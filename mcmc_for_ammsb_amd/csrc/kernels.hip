@@ -1356,6 +1356,13 @@ __global__ void k_fill_f64(double* p, double v, int64_t n) {
        i += (int64_t)gridDim.x * blockDim.x)
     p[i] = v;
 }
+__global__ void k_set_ring(DevState* st, int64_t base, int32_t slots) {
+  if (threadIdx.x == 0) {
+    st->ring_base = base;
+    st->ring_slots = slots;
+  }
+}
+
 __global__ void k_set_step(DevState* st, int64_t step) {
   if (threadIdx.x == 0) {
     st->step = step;
@@ -1588,6 +1595,11 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s) {
 hipError_t launch_fill_f64(double* p, double v, int64_t n, hipStream_t s) {
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(4096, (n + 255) / 256));
   hipLaunchKernelGGL(k_fill_f64, dim3(blocks), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_ring(DevState* st, int64_t base, int32_t slots, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_ring, dim3(1), dim3(64), 0, s, st, base, slots);
   return hipGetLastError();
 }
 
