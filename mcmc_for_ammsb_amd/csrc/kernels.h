@@ -335,8 +335,6 @@ hipError_t launch_batch_index(const PhiArgs* args, const BatchPhiShape* shapes,
                               const int32_t* cfg, int n, BatchIndex ix, hipStream_t s);
 hipError_t launch_update_phi_batch(const PhiArgs* args, const BatchPhiShape* shapes, int n,
                                    BatchIndex ix, int precision, int grid, hipStream_t s);
-hipError_t launch_scatter_batch(const ScatterArgs* args, int n, BatchIndex ix, int precision,
-                                int grid, hipStream_t s);
 // update_phi of the batch in two launches: the persistent gather writes each
 // node's gradient sums to its staging row, the finish applies the phi step
 // and writes pi in place

@@ -933,8 +933,9 @@ __global__ __launch_bounds__(256) void k_phi_finish(PhiArgs a) {
 }
 
 // Publish staged rows: pi[nodes[slot]] = staging[slot] (K+1 elements).
+// One wavefront per staged row, four rows per workgroup.
 template <typename F>
-__device__ __forceinline__ void scatter_body(const ScatterArgs& a, const int slot) {
+__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   int count;
   const int32_t* nodes;
   if (a.use_ring) {
@@ -945,19 +946,14 @@ __device__ __forceinline__ void scatter_body(const ScatterArgs& a, const int slo
     count = a.count_host;
     nodes = a.nodes;
   }
+  const int slot = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (slot >= count) return;
   constexpr int V = VT<F>::V;
-  const int nv = (a.K + 1 + V - 1) / V;
-  const F* src = (const F*)a.staging + (int64_t)slot * a.ld;
-  F* dst = (F*)a.pi + (int64_t)nodes[slot] * a.ld;
+  const int nv = (a.K + 1 + V - 1) / V;  // row and phi_sum; ld is a multiple of 128 B
   typedef typename VT<F>::type T;
-  for (int i = threadIdx.x; i < nv; i += blockDim.x)
-    reinterpret_cast<T*>(dst)[i] = reinterpret_cast<const T*>(src)[i];
-}
-
-template <typename F>
-__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
-  scatter_body<F>(a, blockIdx.x);
+  const T* src = reinterpret_cast<const T*>((const F*)a.staging + (int64_t)slot * a.ld);
+  T* dst = reinterpret_cast<T*>((F*)a.pi + (int64_t)nodes[slot] * a.ld);
+  for (int i = lane; i < nv; i += 64) dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1120,21 +1116,24 @@ __global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
 }
 
 // theta step (mcmc_sampler_stochastic.cc:245-266).  Workgroup b owns
-// k in [64b, 64b+64); its 16 wavefronts sum strided partial rows, then the
-// 16 sums meet in LDS in a fixed order (deterministic).  The last workgroup
-// to finish advances step_count, so every workgroup reads the same step.
+// k in [kThetaK*b, kThetaK*(b+1)); its 1024/kThetaK lane rows sum strided
+// partial rows, then the row sums meet in LDS in a fixed order
+// (deterministic).  The last workgroup to finish advances step_count, so
+// every workgroup reads the same step.
 constexpr int kThetaThreads = 1024;
+constexpr int kThetaK = 64;                        // k per workgroup
+constexpr int kThetaRows = kThetaThreads / kThetaK;  // partial-row strides
 __device__ __forceinline__ void theta_body(const ThetaArgs& a, const int bx, const int gx) {
-  __shared__ double s_g[kThetaThreads / 64][2][64];
+  __shared__ double s_g[kThetaRows][2][kThetaK];
   DevState* st = a.st;
   const int64_t t = st->step;
-  const int tid = threadIdx.x, kl = tid & 63, rg = tid >> 6;
+  const int tid = threadIdx.x, kl = tid % kThetaK, rg = tid / kThetaK;
   const int K = a.K;
-  const int k = bx * 64 + kl;
+  const int k = bx * kThetaK + kl;
   double G0 = 0.0, G1 = 0.0;
   if (k < K) {
 #pragma unroll 8
-    for (int b = rg; b < a.blocks; b += kThetaThreads / 64) {
+    for (int b = rg; b < a.blocks; b += kThetaRows) {
       G0 += a.partials[(int64_t)b * 2 * K + k];
       G1 += a.partials[(int64_t)b * 2 * K + K + k];
     }
@@ -1145,7 +1144,7 @@ __device__ __forceinline__ void theta_body(const ThetaArgs& a, const int bx, con
   if (rg == 0 && k < K) {
     G0 = 0.0;
     G1 = 0.0;
-    for (int q = 0; q < kThetaThreads / 64; ++q) {
+    for (int q = 0; q < kThetaRows; ++q) {
       G0 += s_g[q][0][kl];
       G1 += s_g[q][1][kl];
     }
@@ -1522,10 +1521,11 @@ hipError_t launch_phi_finish(const PhiArgs& a, int precision, int32_t grid_nodes
 hipError_t launch_scatter(const ScatterArgs& a, int precision, int32_t grid_nodes,
                           hipStream_t s) {
   if (grid_nodes <= 0) return hipSuccess;
+  const dim3 grid((grid_nodes + 3) / 4);
   if (precision == 0)
-    hipLaunchKernelGGL(k_scatter<double>, dim3(grid_nodes), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_scatter<double>, grid, dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL(k_scatter<float>, dim3(grid_nodes), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_scatter<float>, grid, dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -1548,7 +1548,7 @@ hipError_t launch_beta(const BetaArgs& a, int precision, hipStream_t s) {
 }
 
 hipError_t launch_theta(const ThetaArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_theta, dim3((a.K + 63) / 64), dim3(kThetaThreads), 0, s, a);
+  hipLaunchKernelGGL(k_theta, dim3((a.K + kThetaK - 1) / kThetaK), dim3(kThetaThreads), 0, s, a);
   return hipGetLastError();
 }
 
@@ -2379,19 +2379,6 @@ __global__ __launch_bounds__(1024) void k_batch_index(const PhiArgs* __restrict_
   }
 }
 
-template <typename F>
-__global__ __launch_bounds__(256) void k_scatter_batch(const ScatterArgs* __restrict__ args, int n,
-                                                       BatchIndex ix) {
-  __shared__ int32_t s_off[kMaxBatchChains + 1];
-  for (int i = threadIdx.x; i <= n; i += blockDim.x) s_off[i] = ix.nodes[i];
-  __syncthreads();
-  const int total = s_off[n];
-  for (int it = blockIdx.x; it < total; it += gridDim.x) {
-    const int c = __builtin_amdgcn_readfirstlane(chain_of(s_off, n, it));
-    scatter_body<F>(args[c], it - __builtin_amdgcn_readfirstlane(s_off[c]));
-  }
-}
-
 // phi step + update_pi of every chain's minibatch nodes: a wavefront takes
 // 64/lpr consecutive nodes of one chain, one per row group of lpr lanes
 // (the chain's update_beta row shape).  Reads the node's row and the
@@ -2484,7 +2471,7 @@ __global__ __launch_bounds__(kBetaWaves * 64) void k_beta_batch(const BetaArgs* 
 
 __global__ __launch_bounds__(kThetaThreads) void k_theta_batch(const ThetaArgs* __restrict__ args) {
   const ThetaArgs& a = args[blockIdx.y];
-  const int gx = (a.K + 63) / 64;
+  const int gx = (a.K + kThetaK - 1) / kThetaK;
   if ((int)blockIdx.x >= gx) return;
   theta_body(a, blockIdx.x, gx);
 }
@@ -2540,16 +2527,6 @@ hipError_t launch_update_phi_batch(const PhiArgs* args, const BatchPhiShape* sha
   return hipGetLastError();
 }
 
-hipError_t launch_scatter_batch(const ScatterArgs* args, int n, BatchIndex ix, int precision,
-                                int grid, hipStream_t s) {
-  if (n < 1 || n > kMaxBatchChains || grid < 1) return hipErrorInvalidValue;
-  if (precision == 0)
-    hipLaunchKernelGGL(k_scatter_batch<double>, dim3(grid), dim3(256), 0, s, args, n, ix);
-  else
-    hipLaunchKernelGGL(k_scatter_batch<float>, dim3(grid), dim3(256), 0, s, args, n, ix);
-  return hipGetLastError();
-}
-
 hipError_t launch_phi_finish_batch(const PhiArgs* args, const int32_t* cfg, int n,
                                    BatchIndex ix, int precision, int grid, hipStream_t s) {
   if (n < 1 || n > kMaxBatchChains || grid < 1) return hipErrorInvalidValue;
@@ -2574,7 +2551,8 @@ hipError_t launch_beta_batch(const BetaArgs* args, const int32_t* cfg, int n, in
 
 hipError_t launch_theta_batch(const ThetaArgs* args, int n, int max_K, hipStream_t s) {
   if (n < 1 || max_K < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_theta_batch, dim3((max_K + 63) / 64, n), dim3(kThetaThreads), 0, s, args);
+  hipLaunchKernelGGL(k_theta_batch, dim3((max_K + kThetaK - 1) / kThetaK, n), dim3(kThetaThreads), 0,
+                     s, args);
   return hipGetLastError();
 }
 
