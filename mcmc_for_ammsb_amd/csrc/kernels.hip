@@ -65,17 +65,40 @@ __device__ __forceinline__ void load_vec<float>(const float* p, float* out) {
   out[3] = v.w;
 }
 
+// Streaming (non-temporal) 16-byte loads, for rows read once per use.
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+template <typename F>
+__device__ __forceinline__ void load_vec_nt(const F* p, F* out);
+template <>
+__device__ __forceinline__ void load_vec_nt<double>(const double* p, double* out) {
+  const nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
+  out[0] = v.x;
+  out[1] = v.y;
+}
+template <>
+__device__ __forceinline__ void load_vec_nt<float>(const float* p, float* out) {
+  const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+  out[0] = v.x;
+  out[1] = v.y;
+  out[2] = v.z;
+  out[3] = v.w;
+}
+
 // Row of K elements into registers: vector vi = c*LPR + lg.  The loads are
 // unconditional so that every vector of a row (and of several rows) is in
 // flight before the first wait: a vector past the end of the row re-reads
 // vector 0 of the same row (in bounds) and is zeroed by mask_row().
-template <typename F, int LPR, int CH>
+template <typename F, int LPR, int CH, bool NT = false>
 __device__ __forceinline__ void issue_row(const F* row, int K, int lg, F* out) {
   constexpr int V = VT<F>::V;
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     const int k0 = (c * LPR + lg) * V;
-    load_vec<F>(row + (k0 < K ? k0 : 0), out + c * V);
+    if constexpr (NT)
+      load_vec_nt<F>(row + (k0 < K ? k0 : 0), out + c * V);
+    else
+      load_vec<F>(row + (k0 < K ? k0 : 0), out + c * V);
   }
 }
 // Zero the elements k >= K (selects, no branches around memory operations).
@@ -840,7 +863,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (LPR == 64) nbv[u] = __builtin_amdgcn_readfirstlane(nbv[u]);  // one row per wavefront
-        issue_row<F, LPR, CH>(pi + (int64_t)(nbv[u] & 0x7fffffff) * a.ld, K, lg, row[u]);
+        issue_row<F, LPR, CH, true>(pi + (int64_t)(nbv[u] & 0x7fffffff) * a.ld, K, lg, row[u]);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -1739,7 +1762,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_loglik(LogLikArgs a) {
       }
       if (b == node) continue;  // mcmc_sampler_stochastic.cc:275-277
       F rb[EPL];
-      load_row<F, LPR, CH>(pi + (int64_t)b * a.ld, K, lg, rb);
+      issue_row<F, LPR, CH, true>(pi + (int64_t)b * a.ld, K, lg, rb);
+      mask_row<F, LPR, CH>(K, lg, rb);
       F z = (F)0;
 #pragma unroll
       for (int j = 0; j < EPL; ++j)
@@ -1873,7 +1897,7 @@ __device__ __forceinline__ void ll_load(const LogLikArgs& a, const F* pi, int32_
 #pragma unroll
     for (int c = 0; c < CHC; ++c) {
       const int k0 = ((w * CHC + c) * 64 + lane) * V;
-      load_vec<F>(row + (k0 < a.K ? k0 : 0), gr.rb[i] + c * V);
+      load_vec_nt<F>(row + (k0 < a.K ? k0 : 0), gr.rb[i] + c * V);
     }
   }
 }
