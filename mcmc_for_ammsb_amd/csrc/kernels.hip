@@ -1268,7 +1268,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_ppx(PpxArgs a) {
         yv[u] = __builtin_amdgcn_readfirstlane(yv[u]);
       }
       issue_row<F, LPR, CH>(pi + (int64_t)ea[u] * a.ld, K, lg, ra[u]);
-      issue_row<F, LPR, CH>(pi + (int64_t)eb[u] * a.ld, K, lg, rb[u]);
+      // row a is shared by neighbouring held-out pairs (sorted by a); row b is streamed
+      issue_row<F, LPR, CH, true>(pi + (int64_t)eb[u] * a.ld, K, lg, rb[u]);
     }
 #pragma unroll
     for (int u = 0; u < UE; ++u) {
