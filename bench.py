@@ -509,8 +509,9 @@ def main():
         try:
             with open(args.traffic) as f:
                 tr = json.load(f)
-            if tr.get("workload") == args.workload:
-                traffic = tr.get("bytes_per_launch")
+            entry = tr.get(args.workload)
+            if isinstance(entry, dict):
+                traffic = entry.get("bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
 
