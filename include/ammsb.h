@@ -307,7 +307,10 @@ int ammsb_get_totals(ammsb_ctx *ctx, int64_t *nodes_updated,
  * phi_unroll is set.  Results do not depend on the shape beyond
  * summation order (the log-likelihood shapes not even on that).  Defaults:
  * update_phi (4, 1); log-likelihood FP64 (4,0) both ways, FP32 forward
- * (8,0), adjoint (2,1). */
+ * (8,0), adjoint (2,1).
+ * ammsb_get_tuning also reads "sampler_lds" (read-only): bit 0 set when the
+ * minibatch sampler's de-dup table lives in LDS, bit 1 when the neighbour
+ * sampler's tables do (else they live in HBM). */
 int ammsb_set_tuning(ammsb_ctx *ctx, const char *name, int64_t value);
 int ammsb_get_tuning(ammsb_ctx *ctx, const char *name, int64_t *value);
 

@@ -191,6 +191,8 @@ DevGraph dev_graph(const ammsb_ctx* c) {
   g.bloom = c->d_bloom;
   g.bloom_mask = c->bloom_mask;
   g.esrc = c->d_esrc;
+  g.n_recip = fastmod_recip((uint32_t)c->N);
+  g.e2_recip = c->E > 0 ? fastmod_recip((uint32_t)(2 * c->E)) : 0;
   return g;
 }
 
@@ -1441,6 +1443,8 @@ int ammsb_get_tuning(ammsb_ctx* c, const char* name, int64_t* value) {
     *value = c->ll_pairs[1];
   else if (n == "ll_bwd_prefetch")
     *value = c->ll_prefetch[1];
+  else if (n == "sampler_lds")  // read-only: bit 0 minibatch table, bit 1 neighbour tables in LDS
+    *value = (c->mbtable_in_lds ? 1 : 0) | (c->nbr_use_lds ? 2 : 0);
   else {
     set_error("ammsb_get_tuning: unknown parameter " + n);
     return AMMSB_ERR_ARG;

@@ -48,6 +48,8 @@ struct DevGraph {
   const uint64_t* bloom;     // blocked Bloom filter of the same keys
   uint64_t bloom_mask;
   const int32_t* esrc;       // [2E] row of each CSR entry (random-edge strategy only)
+  uint64_t n_recip;          // fastmod_recip(N): randint(0, N-1) without a division
+  uint64_t e2_recip;         // fastmod_recip(2E)
 };
 
 // Hyper-parameters as the kernels see them (all double; each kernel casts

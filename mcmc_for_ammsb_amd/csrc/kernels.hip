@@ -260,6 +260,17 @@ __device__ __forceinline__ int table_insert(int2* tab, uint32_t cap_mask, int32_
   }
 }
 
+// Hash set of node ids (-1 = empty): true when v was not in the set.
+__device__ __forceinline__ bool set_insert(int* set, uint32_t cap_mask, int32_t v) {
+  uint32_t s = (uint32_t)mix64((uint64_t)(uint32_t)v) & cap_mask;
+  while (true) {
+    const int prev = atomicCAS(&set[s], -1, v);
+    if (prev == -1) return true;
+    if (prev == v) return false;
+    s = (s + 1) & cap_mask;
+  }
+}
+
 // De-dup table of 64-bit pair keys (random-edge minibatches).
 __device__ __forceinline__ int table_insert64(unsigned long long* keys, uint32_t cap_mask,
                                               unsigned long long k) {
@@ -375,7 +386,8 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
     s_err = 0;
   }
   if (flag == 0) {
-    const int32_t node = (int32_t)randint_mod(draw_u64(a.key, 0, 0, t32, TAG_SOURCE), N);
+    const int32_t node =
+        (int32_t)randint_fast(draw_u64(a.key, 0, 0, t32, TAG_SOURCE), N, a.g.n_recip);
     const int64_t rs = a.g.row_ptr[node];
     const int64_t deg = a.g.row_ptr[node + 1] - rs;
     const bool in_lds = deg <= kRowLds;
@@ -393,7 +405,8 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
         break;
       }
       const uint32_t ai = (uint32_t)(batch * kMbThreads + tid);
-      const int32_t v = (int32_t)randint_mod(draw_u64(a.key, ai, 0, t32, TAG_NONLINK), N);
+      const int32_t v =
+          (int32_t)randint_fast(draw_u64(a.key, ai, 0, t32, TAG_NONLINK), N, a.g.n_recip);
       bool valid = v != node;
       const uint32_t lo = (uint32_t)min(node, v), hi = (uint32_t)max(node, v);
       if (valid) valid = !(in_lds ? row_contains(s_row, deg, v) : row_contains(a.g.cols + rs, deg, v));
@@ -413,9 +426,15 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
         mb_edges[2 * (c + pos)] = (int32_t)lo;
         mb_edges[2 * (c + pos) + 1] = (int32_t)hi;
         mb_y[c + pos] = 0;
+        // node set: the source, then the (distinct) partners in order
+        if (1 + c + pos < node_cap) out_nodes[1 + c + pos] = v;
       }
       c = min(m, c + (int64_t)total);
       if (c >= m) break;
+    }
+    if (tid == 0) {
+      if (c > 0) out_nodes[0] = node;
+      s_nodes = c > 0 ? (int)c + 1 : 0;
     }
     const double dN = (double)a.g.N, dE = (double)a.g.E;
     scale = dN * dN / (double)m;
@@ -464,8 +483,8 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
           break;
         }
         const uint32_t ai = (uint32_t)(batch * kMbThreads + tid);
-        const int64_t q = (int64_t)randint_mod(draw_u64(a.key, ai, 0, t32, TAG_RANDOM_EDGE),
-                                               (uint32_t)E2);
+        const int64_t q = (int64_t)randint_fast(draw_u64(a.key, ai, 0, t32, TAG_RANDOM_EDGE),
+                                                (uint32_t)E2, a.g.e2_recip);
         const int32_t u = a.g.esrc[q], v = a.g.cols[q];
         const uint32_t lo = (uint32_t)min(u, v), hi = (uint32_t)max(u, v);
         const bool valid = !pair_held_out(a.g, lo, hi);
@@ -500,7 +519,8 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
       __syncthreads();
       for (int s = tid; s < nsrc; s += kMbThreads) {
         const int32_t node =
-            (int32_t)randint_mod(draw_u64(a.key, (uint32_t)r, (uint32_t)s, t32, TAG_SOURCE), N);
+            (int32_t)randint_fast(draw_u64(a.key, (uint32_t)r, (uint32_t)s, t32, TAG_SOURCE), N,
+                                  a.g.n_recip);
         s_src[s] = node;
         s_deg[s] = (int)(a.g.row_ptr[node + 1] - a.g.row_ptr[node]);
         const int slot = table_insert(s_stab, kSrcTab - 1, node);
@@ -599,9 +619,19 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
   __syncthreads();
   // nodes_in_batch: unique endpoints (mcmc_sampler_stochastic.cc:315-327).
   // Order is not part of the contract: every per-node draw is keyed by the
-  // node id, not by its position.
+  // node id, not by its position.  Non-link minibatches wrote theirs while
+  // sampling; for the others a node is new when it enters an LDS hash set
+  // (the non-link table's space, at load <= 3/4), else when it swaps the
+  // slot's stamp generation into its global stamp.
   const uint32_t epoch = R.meta[slot].epoch + 1u;
-  for (int64_t e0 = 0; e0 < c; e0 += kMbThreads) {
+  const int64_t set_ints = 2 * a.table_cap;
+  const bool lds_set = a.table_in_lds && 3 * set_ints >= 8 * c;
+  int* const nset = reinterpret_cast<int*>(s_dyn);
+  if (flag != 0 && lds_set) {
+    for (int64_t i = tid; i < set_ints; i += kMbThreads) nset[i] = -1;
+    __syncthreads();
+  }
+  for (int64_t e0 = 0; flag != 0 && e0 < c; e0 += kMbThreads) {
     const int64_t e = e0 + tid;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -609,7 +639,8 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
       bool isnew = false;
       if (e < c) {
         v = mb_edges[2 * e + q];
-        isnew = atomicExch(&stamp[v], epoch) != epoch;
+        isnew = lds_set ? set_insert(nset, (uint32_t)(set_ints - 1), v)
+                        : atomicExch(&stamp[v], epoch) != epoch;
       }
       // one LDS atomic per wavefront, positions by ballot prefix
       const uint64_t mk = __ballot(isnew);
@@ -690,7 +721,8 @@ __device__ __forceinline__ void neighbors_body(const NeighborArgs& a, int32_t sl
       break;
     }
     const uint32_t ai = (uint32_t)(batch * 64 + lane);
-    const int32_t v = (int32_t)randint_mod(draw_u64(a.key, ai, (uint32_t)node, t32, TAG_NEIGHBOR), N);
+    const int32_t v = (int32_t)randint_fast(draw_u64(a.key, ai, (uint32_t)node, t32, TAG_NEIGHBOR), N,
+                                         a.g.n_recip);
     bool valid = v != node;
     if (valid) valid = !pair_held_out(a.g, (uint32_t)min(node, v), (uint32_t)max(node, v));
     int s = -1;
@@ -737,7 +769,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_neighbors(NeighborArgs a, int32_
 // LPR = 0: the row group has lpr_rt lanes (a uniform runtime value).
 template <typename F, int LPR, int CH>
 __device__ __forceinline__ void phi_epilogue(const PhiArgs& a, int slot, int32_t node,
-                                             const F* prow, const F* pii, const F* acc, int lg,
+                                             const F phi_sum, const F* pii, const F* acc, int lg,
                                              F* stg, int lpr_rt = 0) {
   constexpr int V = VT<F>::V;
   constexpr int EPL = V * CH;
@@ -749,7 +781,6 @@ __device__ __forceinline__ void phi_epilogue(const PhiArgs& a, int slot, int32_t
   const F half_eps = eps_t / (F)2;
   const F alpha = (F)a.hp.alpha;
   const F Nn = (F)a.hp.Nn;
-  const F phi_sum = prow[K];
   const F inv_phi_sum = recip(phi_sum);
   const F* noise = (const F*)a.noise;
   F phin[EPL];
@@ -830,6 +861,18 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
   const int32_t node = nodes[slot];
   const F* prow = pi + (int64_t)node * a.ld;
   const F eps = (F)a.hp.epsilon;
+  const int32_t* nb_src = nbrs + (int64_t)slot * a.nnb;
+  // Up to 64*WAVES neighbours (uniform test): each wavefront holds its row
+  // ids in one register, lane j the id of row w*RPW + j%RPW + STRIDE*(j/RPW),
+  // read together with the node's own row and phi_sum (no LDS staging and no
+  // barrier before the first neighbour row is requested).
+  const bool reg_ids = a.nnb <= 64 * WAVES;
+  int32_t ids = node;
+  if (reg_ids) {
+    const int r = w * RPW + lane % RPW + STRIDE * (lane / RPW);
+    if (r < a.nnb) ids = nb_src[r];
+  }
+  const F phi_sum = prow[K];
 
   F pii[EPL], f1[EPL], acc[EPL];
   load_row<F, LPR, CH>(prow, K, lg, pii);
@@ -843,22 +886,35 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
       acc[c * V + v] = (F)0;
     }
   const F e1 = eps, e0 = (F)1 - eps;
-  // The node's neighbour ids (bit 31 = link) are staged in LDS, chunk by
-  // chunk, and the U ids of an iteration are read before any of its rows is
-  // requested, so that the U rows' loads are all in flight together.
-  const int32_t* nb_src = nbrs + (int64_t)slot * a.nnb;
+  // Otherwise the node's neighbour ids (bit 31 = link) are staged in LDS,
+  // chunk by chunk.  The U ids of an iteration are read before any of its
+  // rows is requested, so that the U rows' loads are all in flight together.
+  // The trip count is uniform across the wavefront (row group 0's), so every
+  // lane takes part in the cross-lane id reads.
   for (int cb = 0; cb < a.nnb; cb += kPhiNbLds) {
     const int cn = min(kPhiNbLds, a.nnb - cb);
-    __syncthreads();
-    for (int i = tid; i < cn; i += WAVES * 64) s_nb[i] = nb_src[cb + i];
-    __syncthreads();
-    for (int b0 = w * RPW + g; b0 < cn; b0 += U * STRIDE) {
+    if (!reg_ids) {
+      __syncthreads();
+      for (int i = tid; i < cn; i += WAVES * 64) s_nb[i] = nb_src[cb + i];
+      __syncthreads();
+    }
+    for (int it = 0; w * RPW + it * STRIDE < cn; it += U) {
+      const int b0 = w * RPW + g + it * STRIDE;
       F row[U][EPL];
       int32_t nbv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int b = b0 + u * STRIDE;
-        nbv[u] = b < cn ? s_nb[b] : node;  // past the end: own row, not counted
+        int32_t v;
+        if (reg_ids) {
+          if constexpr (RPW == 1)
+            v = __builtin_amdgcn_readlane(ids, min(it + u, 63));
+          else
+            v = __shfl(ids, ((it + u) * RPW + g) & 63, 64);
+        } else {
+          v = b < cn ? s_nb[b] : node;
+        }
+        nbv[u] = b < cn ? v : node;  // past the end: own row, not counted
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -911,7 +967,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
   if (g != 0) return;
 
   if constexpr (EPI) {
-    phi_epilogue<F, LPR, CH>(a, slot, node, prow, pii, acc, lg,
+    phi_epilogue<F, LPR, CH>(a, slot, node, phi_sum, pii, acc, lg,
                              (F*)a.staging + (int64_t)slot * a.ld);
   } else {
     F* stg = (F*)a.staging + (int64_t)slot * a.ld;
@@ -952,7 +1008,7 @@ __global__ __launch_bounds__(256) void k_phi_finish(PhiArgs a) {
   F pii[EPL], acc[EPL];
   load_row<F, LPR, CH>(prow, a.K, lg, pii);
   load_row<F, LPR, CH>((const F*)a.staging + (int64_t)slot * a.ld, a.K, lg, acc);
-  phi_epilogue<F, LPR, CH>(a, slot, node, prow, pii, acc, lg, prow);
+  phi_epilogue<F, LPR, CH>(a, slot, node, prow[a.K], pii, acc, lg, prow);
 }
 
 // Publish staged rows: pi[nodes[slot]] = staging[slot] (K+1 elements).
@@ -2435,7 +2491,7 @@ __device__ __forceinline__ void phi_finish_body(const PhiArgs& a, int slot, int 
       pii[c * V + v] = in ? pii[c * V + v] : (F)0;
       acc[c * V + v] = in ? acc[c * V + v] : (F)0;
     }
-  phi_epilogue<F, 0, CH>(a, slot, node, prow, pii, acc, lg, prow, lpr);
+  phi_epilogue<F, 0, CH>(a, slot, node, prow[a.K], pii, acc, lg, prow, lpr);
 }
 
 template <typename F>

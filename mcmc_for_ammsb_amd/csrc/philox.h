@@ -107,6 +107,22 @@ AMMSB_HD uint32_t randint_mod(uint64_t u, uint32_t n) {
   return (uint32_t)(u % (uint64_t)n);
 }
 
+// The same u % n without a 64-bit division (Barrett reduction with the
+// precomputed r = floor((2^64-1)/n), n >= 1): q = floor(u*r / 2^64) is
+// floor(u/n) or one less, because u*r/2^64 > u/n - (1 + (2^64-1) mod n)/n
+// >= u/n - 1; so u - q*n < 2n and one conditional subtraction is exact.
+AMMSB_HD uint64_t fastmod_recip(uint32_t n) { return ~0ull / (uint64_t)n; }
+AMMSB_HD uint32_t randint_fast(uint64_t u, uint32_t n, uint64_t r) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint64_t q = __umul64hi(u, r);
+#else
+  const uint64_t q = (uint64_t)(((unsigned __int128)u * r) >> 64);
+#endif
+  uint64_t rem = u - q * (uint64_t)n;
+  if (rem >= (uint64_t)n) rem -= (uint64_t)n;
+  return (uint32_t)rem;
+}
+
 // Uniform doubles in the open interval (0, 1): 53 high bits + 1/2 ulp.
 AMMSB_HD double u01_from_u64(uint64_t u) {
   return ((double)(u >> 11) + 0.5) * (1.0 / 9007199254740992.0);
