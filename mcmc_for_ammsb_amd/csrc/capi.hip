@@ -180,7 +180,7 @@ uint64_t pow2_at_least(uint64_t x) {
 }
 
 DevGraph dev_graph(const ammsb_ctx* c) {
-  DevGraph g;
+  DevGraph g{};
   g.N = c->N;
   g.E = c->E;
   g.row_ptr = c->d_row_ptr;
@@ -246,7 +246,7 @@ struct PhaseScope {
 
 // ---- kernel arguments of one iteration, built from the context
 SamplerArgs sampler_args(const ammsb_ctx* c, int64_t t0) {
-  SamplerArgs sa;
+  SamplerArgs sa{};
   sa.g = dev_graph(c);
   sa.st = c->d_st;
   sa.key = c->key;
@@ -266,7 +266,7 @@ SamplerArgs sampler_args(const ammsb_ctx* c, int64_t t0) {
 }
 
 NeighborArgs neighbor_args(const ammsb_ctx* c) {
-  NeighborArgs na;
+  NeighborArgs na{};
   na.g = dev_graph(c);
   na.st = c->d_st;
   na.key = c->key;
@@ -279,7 +279,7 @@ NeighborArgs neighbor_args(const ammsb_ctx* c) {
 }
 
 PhiArgs phi_args(const ammsb_ctx* c) {
-  PhiArgs pa;
+  PhiArgs pa{};
   pa.pi = c->d_pi;
   pa.ld = c->ld;
   pa.K = c->K;
@@ -301,7 +301,7 @@ PhiArgs phi_args(const ammsb_ctx* c) {
 }
 
 ScatterArgs scatter_args(const ammsb_ctx* c) {
-  ScatterArgs sc;
+  ScatterArgs sc{};
   sc.pi = c->d_pi;
   sc.ld = c->ld;
   sc.K = c->K;
@@ -315,7 +315,7 @@ ScatterArgs scatter_args(const ammsb_ctx* c) {
 }
 
 BetaArgs beta_args(const ammsb_ctx* c) {
-  BetaArgs ba;
+  BetaArgs ba{};
   ba.pi = c->d_pi;
   ba.ld = c->ld;
   ba.K = c->K;
@@ -334,7 +334,7 @@ BetaArgs beta_args(const ammsb_ctx* c) {
 }
 
 ThetaArgs theta_args(const ammsb_ctx* c) {
-  ThetaArgs ta;
+  ThetaArgs ta{};
   ta.theta = c->d_theta;
   ta.beta = c->d_beta;
   ta.K = c->K;
@@ -486,7 +486,7 @@ int check_devstate(ammsb_ctx* c) {
 }
 
 int do_perplexity(ammsb_ctx* c, double* out) {
-  PpxArgs p;
+  PpxArgs p{};
   p.pi = c->d_pi;
   p.ld = c->ld;
   p.K = c->K;
@@ -1208,7 +1208,7 @@ int ammsb_update_phi(ammsb_ctx* c, const int32_t* nodes, int64_t n_nodes, const 
     e = launch_label_nbrs(dev_graph(c), d_nodes, (int32_t)n_nodes, d_nbrs, (int32_t)n_nb, c->stream);
   bool split = false;
   if (e == hipSuccess) {
-    PhiArgs pa;
+    PhiArgs pa{};
     pa.pi = c->d_pi;
     pa.ld = c->ld;
     pa.K = c->K;
@@ -1232,7 +1232,7 @@ int ammsb_update_phi(ammsb_ctx* c, const int32_t* nodes, int64_t n_nodes, const 
     if (e == hipSuccess && split) e = launch_phi_finish(pa, c->prec, (int32_t)n_nodes, c->stream);
   }
   if (e == hipSuccess && !split) {
-    ScatterArgs sc;
+    ScatterArgs sc{};
     sc.pi = c->d_pi;
     sc.ld = c->ld;
     sc.K = c->K;
@@ -1277,7 +1277,7 @@ int ammsb_update_beta(ammsb_ctx* c, const int32_t* edges, const uint8_t* y, int6
     if (e == hipSuccess && noise)
       e = hipMemcpy(d_noise, noise, 16 * c->K, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-      BetaArgs ba;
+      BetaArgs ba{};
       ba.pi = c->d_pi;
       ba.ld = c->ld;
       ba.K = c->K;
@@ -1295,7 +1295,7 @@ int ammsb_update_beta(ammsb_ctx* c, const int32_t* edges, const uint8_t* y, int6
       e = launch_beta(ba, c->prec, c->stream);
     }
     if (e == hipSuccess) {
-      ThetaArgs ta;
+      ThetaArgs ta{};
       ta.theta = c->d_theta;
       ta.beta = c->d_beta;
       ta.K = c->K;
@@ -1504,7 +1504,7 @@ int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi
   }
   int32_t* d_err = (int32_t*)(c->d_ll_scratch + std::max<int64_t>(M, (int64_t)blocks * c->K) + 1);
   HIP_TRY(launch_fill_u32((uint32_t*)d_err, 0u, 1, s));  // a kernel, not a memset node
-  LogLikArgs a;
+  LogLikArgs a{};
   a.pi = c->d_pi;
   a.ld = c->ld;
   a.K = c->K;
