@@ -370,8 +370,10 @@ def run_batch(args):
                          for sh in shapes]
     views = [b.chain(i) for i in range(len(b))]
     res = {}
-    for mode, graph in (("graph", True), ("direct", False)):
+    for mode, graph, in_order in (("graph", True, True), ("graph_refill_overlapped", True, False),
+                                  ("direct", False, True)):
         b.use_graph(graph)
+        b.refill_in_order(in_order)
         b.step(2)
         n0 = totals(views)
         t0 = time.perf_counter()

@@ -183,6 +183,7 @@ def _load():
         "ammsb_batch_step": (c_i32, [c_vp, c_i64]),
         "ammsb_batch_perplexity": (c_i32, [c_vp, P_dbl]),
         "ammsb_batch_use_graph": (c_i32, [c_vp, c_i32]),
+        "ammsb_batch_refill_in_order": (c_i32, [c_vp, c_i32]),
         "ammsb_batch_set_shape": (c_i32, [c_vp, c_i32, ctypes.POINTER(BatchShape)]),
         "ammsb_batch_get_shape": (c_i32, [c_vp, c_i32, ctypes.POINTER(BatchShape)]),
         "ammsb_batch_autotune": (c_i32, [c_vp, c_i32, ctypes.POINTER(BatchShape)]),

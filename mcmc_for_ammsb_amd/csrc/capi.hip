@@ -1612,6 +1612,7 @@ struct ammsb_batch {
   int32_t max_node_cap = 0, max_beta_blocks = 0, max_K = 0;
   int32_t phi_grid = 1, finish_grid = 1;
   bool use_graph = true;
+  bool refill_in_order = true;  // sampler refills on the main stream (else a second stream)
   hipGraph_t graph = nullptr;  // one iteration (samplers excluded)
   hipGraphExec_t exec = nullptr;
 };
@@ -1950,6 +1951,12 @@ int ammsb_batch_use_graph(ammsb_batch* b, int32_t enable) {
   return AMMSB_OK;
 }
 
+int ammsb_batch_refill_in_order(ammsb_batch* b, int32_t enable) {
+  if (!b) return AMMSB_ERR_ARG;
+  b->refill_in_order = enable != 0;
+  return AMMSB_OK;
+}
+
 int ammsb_batch_step(ammsb_batch* b, int64_t iters) {
   if (!b || iters < 0) return AMMSB_ERR_ARG;
   for (auto* c : b->chains)
@@ -1976,7 +1983,18 @@ int ammsb_batch_step(ammsb_batch* b, int64_t iters) {
   const int64_t R = b->R;
   for (int64_t t = S; t < S + iters; ++t) {
     const int64_t k = (t - b->ring_B) / R, pos = (t - b->ring_B) % R;
-    if (pos == 0) {
+    if (pos == 0 && b->refill_in_order) {
+      // block k+1 drawn in stream order, into the half block k-1 used
+      // (block k was drawn before it, in stream order or by the event wait
+      // of an overlapped refill)
+      HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_fill[k % 2], 0));
+      if (b->ring_hi == b->ring_B + (k + 1) * R) {
+        const int rc = batch_enqueue_fill(b, k + 1, b->stream);
+        if (rc != AMMSB_OK) return rc;
+        HIP_TRY(hipEventRecord(b->ev_fill[(k + 1) % 2], b->stream));
+        b->ring_hi += R;
+      }
+    } else if (pos == 0) {
       // block k's draws are complete; block k+1 is drawn on the sampler
       // stream while block k computes, into the half block k-1 used
       HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_fill[k % 2], 0));

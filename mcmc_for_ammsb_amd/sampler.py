@@ -316,6 +316,11 @@ class MCMCBatch:
     def use_graph(self, enable=True):
         _lib.check(_lib.lib.ammsb_batch_use_graph(self._h, 1 if enable else 0))
 
+    def refill_in_order(self, enable=True):
+        """Draw the ring's next 16 iterations on the batch's own stream (in
+        order) instead of overlapping them on a second stream."""
+        _lib.check(_lib.lib.ammsb_batch_refill_in_order(self._h, 1 if enable else 0))
+
     def get_shape(self, i):
         sh = _lib.BatchShape()
         _lib.check(_lib.lib.ammsb_batch_get_shape(self._h, int(i), ctypes.byref(sh)))
