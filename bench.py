@@ -103,6 +103,24 @@ def cpu_baseline(g, s, opts, pi, theta, start_step, budget_s, threads):
     }
 
 
+def host_info():
+    """Host cores and CPU model the CPU baseline ran on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = None
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model}
+
+
 def trajectory(smp, steps, interval):
     """Checkpoints (step, wall s including perplexity, wall s excluding it, perplexity),
     perplexity before the first step and then every `interval` steps
@@ -560,7 +578,13 @@ def main():
         pi, theta, _ = smp.get_state()
         step = smp.info["step_count"]
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(g, s, opts, pi, theta, step, args.cpu_budget, threads)
+        cb = cpu_baseline(g, s, opts, pi, theta, step, args.cpu_budget, threads)
+        # the reference's SEQ build runs the loop on one thread
+        seq = cpu_baseline(g, s, opts, pi, theta, step, args.cpu_budget / 2, 1)
+        cb["single_thread"] = {k: seq[k] for k in ("value", "unit", "cores", "sample",
+                                                   "update_phi_node_k_per_s", "seconds")}
+        cb["host"] = host_info()
+        out["cpu_baseline"] = cb
         del pi
     smp.close()
     if rank == 0:
