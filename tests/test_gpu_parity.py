@@ -67,7 +67,7 @@ def make_pair(kind, N, K, precision="fp64", n=32, m=None, strategy="stratified-r
 @pytest.mark.parametrize("split", [0, 1], ids=["fused", "split"])
 @pytest.mark.parametrize("kind,N,K", [("planted", 2000, 32), ("community", 20000, 256),
                                       ("community", 8000, 300), ("community", 4000, 7),
-                                      ("community", 3000, 1024)])
+                                      ("community", 3000, 1024), ("community", 3000, 1)])
 def test_update_phi_fp64_injected(kind, N, K, split):
     """split: gather kernel + k_phi_finish instead of the fused step."""
     print(f"update_phi fp64 {kind} N={N} K={K} split={split}", flush=True)
@@ -93,6 +93,33 @@ def test_update_phi_fp64_injected(kind, N, K, split):
     assert ok, rel
     untouched = np.setdiff1d(np.arange(N), nodes)
     assert np.array_equal(gpi[untouched], pi[untouched])
+
+
+@pytest.mark.parametrize("nnb", [25, 301])
+@pytest.mark.parametrize("waves,unroll", [(4, 1), (2, 2), (2, 4), (4, 2), (4, 4), (8, 1), (8, 2),
+                                          (8, 4)])
+def test_update_phi_launch_shapes(waves, unroll, nnb):
+    """Every update_phi launch shape (wavefronts per node, rows in flight)
+    against the oracle, with neighbour lists that fit the wavefronts' id
+    registers (25) and that are staged in LDS (301 > 64 x wavefronts for 2
+    and 4 wavefronts)."""
+    print(f"update_phi shape waves={waves} unroll={unroll} nnb={nnb}", flush=True)
+    N, K = 6000, 256
+    g, s, smp, orc = make_pair("community", N, K, n=nnb - 1)
+    smp.set_tuning(phi_waves=waves, phi_unroll=unroll)
+    pi, theta = _state(N, K, 17)
+    smp.set_state(pi, theta)
+    orc.set_state(pi, theta)
+    rng = np.random.default_rng(19)
+    nodes = rng.choice(N, size=150, replace=False).astype(np.int32)
+    nbrs = np.stack([orc.sample_neighbors(int(v), nnb, 42, 7) & 0x7FFFFFFF for v in nodes])
+    noise = rng.standard_normal((len(nodes), K))
+    smp.update_phi(nodes, nbrs, noise, 0.011)
+    orc.update_phi(nodes, nbrs, noise, 0.011)
+    gpi, _, _ = smp.get_state()
+    opi, _, _ = orc.get_state()
+    ok, rel = _close(gpi[nodes], opi[nodes], TOL64)
+    assert ok, rel
 
 
 @pytest.mark.parametrize("K", [32, 1024])
