@@ -1196,12 +1196,12 @@ __global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
 
 // theta step (mcmc_sampler_stochastic.cc:245-266).  Workgroup b owns
 // k in [kThetaK*b, kThetaK*(b+1)); its 1024/kThetaK lane rows sum strided
-// partial rows, then the row sums meet in LDS in a fixed order
-// (deterministic).  The last workgroup to finish advances step_count, so
-// every workgroup reads the same step.
+// partial rows, then the row sums meet in LDS by a pairwise tree in a fixed
+// order (deterministic).  The last workgroup to finish advances step_count,
+// so every workgroup reads the same step.
 constexpr int kThetaThreads = 1024;
-constexpr int kThetaK = 64;                        // k per workgroup
-constexpr int kThetaRows = kThetaThreads / kThetaK;  // partial-row strides
+constexpr int kThetaK = 16;                        // k per workgroup
+constexpr int kThetaRows = kThetaThreads / kThetaK;  // partial-row strides (power of two)
 __device__ __forceinline__ void theta_body(const ThetaArgs& a, const int bx, const int gx) {
   __shared__ double s_g[kThetaRows][2][kThetaK];
   DevState* st = a.st;
@@ -1220,13 +1220,17 @@ __device__ __forceinline__ void theta_body(const ThetaArgs& a, const int bx, con
   s_g[rg][0][kl] = G0;
   s_g[rg][1][kl] = G1;
   __syncthreads();
-  if (rg == 0 && k < K) {
-    G0 = 0.0;
-    G1 = 0.0;
-    for (int q = 0; q < kThetaRows; ++q) {
-      G0 += s_g[q][0][kl];
-      G1 += s_g[q][1][kl];
+#pragma unroll
+  for (int h = kThetaRows / 2; h >= 1; h >>= 1) {
+    if (rg < h) {
+      s_g[rg][0][kl] += s_g[rg + h][0][kl];
+      s_g[rg][1][kl] += s_g[rg + h][1][kl];
     }
+    __syncthreads();
+  }
+  if (rg == 0 && k < K) {
+    G0 = s_g[0][0][kl];
+    G1 = s_g[0][1][kl];
     const double eps_t = a.eps_override >= 0.0 ? a.eps_override : eps_of_step(a.hp, t);
     const double scale =
         a.scale_override >= 0.0 ? a.scale_override : a.meta[ring_slot(st)].scale;
