@@ -1081,9 +1081,9 @@ __device__ __forceinline__ void beta_body(const BetaArgs& a, const int bx) {
       g0[j] = 0.0;
       g1[j] = 0.0;
     }
-  // Two edges per row group in flight: both edges' ids are read, then all
-  // four rows are requested, then the edges are reduced.
-  constexpr int UE = CH <= 4 ? 2 : 1;  // big rows: one edge at a time (registers)
+  // Several edges per row group in flight: the edges' ids are read, then
+  // all their rows are requested, then the edges are reduced.
+  constexpr int UE = CH <= 2 ? 4 : (CH <= 4 ? 2 : 1);  // fewer for big rows (registers)
   const int gstride = a.blocks * WAVES * RPW;
   for (int e0 = (bx * WAVES + w) * RPW + g; e0 < count; e0 += UE * gstride) {
     int32_t ea[UE], eb[UE];
