@@ -297,6 +297,9 @@ int ammsb_get_totals(ammsb_ctx *ctx, int64_t *nodes_updated,
  *                wider than 2 KB): 2, 4, 8
  *   "ll_fwd_prefetch", "ll_bwd_prefetch"  1: the next pairs' rows are
  *                loaded before the exchange
+ *   "ll_fwd_rows" 1 (default): the forward pass of rows wider than 2 KB
+ *                streams whole rows per wavefront against the source row
+ *                held in LDS; 0: the K-chunked kernel above
  *   "phi_split"  1: update_phi as a gather kernel + a phi-step kernel
  *                instead of the fused step + scatter (default 0)
  *   "async_ring" 1: the minibatch/neighbour samplers draw the next 16

@@ -120,6 +120,7 @@ struct ammsb_ctx {
   int32_t phi_split = 0;  // 1: gather + finish kernels instead of fused step + scatter
   // K-chunked log-likelihood launch shapes [0] forward, [1] adjoint
   int32_t ll_pairs[2] = {0, 0}, ll_prefetch[2] = {0, 0};
+  int32_t ll_fwd_rows = 1;  // forward of wide rows: whole rows per wavefront (else K-chunked)
   double* d_ll_scratch = nullptr;   // differentiation: per-source values / dbeta partials
   size_t ll_scratch_bytes = 0;
   int2* d_nbr_gtable = nullptr;
@@ -1390,6 +1391,11 @@ int ammsb_set_tuning(ammsb_ctx* c, const char* name, int64_t value) {
     c->ring_valid = false;
     return AMMSB_OK;
   }
+  if (n == "ll_fwd_rows") {
+    if (value != 0 && value != 1) return AMMSB_ERR_ARG;
+    c->ll_fwd_rows = (int32_t)value;
+    return AMMSB_OK;
+  }
   if (n == "ll_fwd_pairs" || n == "ll_fwd_prefetch" || n == "ll_bwd_pairs" ||
       n == "ll_bwd_prefetch") {
     const int d = n.compare(0, 6, "ll_bwd") == 0 ? 1 : 0;
@@ -1443,6 +1449,8 @@ int ammsb_get_tuning(ammsb_ctx* c, const char* name, int64_t* value) {
     *value = c->ll_pairs[1];
   else if (n == "ll_bwd_prefetch")
     *value = c->ll_prefetch[1];
+  else if (n == "ll_fwd_rows")
+    *value = c->ll_fwd_rows;
   else if (n == "sampler_lds")  // read-only: bit 0 minibatch table, bit 1 neighbour tables in LDS
     *value = (c->mbtable_in_lds ? 1 : 0) | (c->nbr_use_lds ? 2 : 0);
   else {
@@ -1526,7 +1534,8 @@ int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi
   a.P = P;
   a.error = d_err;
   if (!bwd) {
-    HIP_TRY(launch_loglik(a, c->prec, false, s, c->ll_pairs[0], c->ll_prefetch[0]));
+    HIP_TRY(launch_loglik(a, c->prec, false, s, c->ll_pairs[0], c->ll_prefetch[0],
+                          c->ll_fwd_rows != 0));
     double* out = total ? total : c->d_ll_scratch + std::max<int64_t>(M, (int64_t)blocks * c->K);
     HIP_TRY(launch_sum(a.per_source, M, out, s));
   } else {
@@ -1534,7 +1543,7 @@ int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi
       set_error("ammsb_loglik_bwd: grad_phi is required");
       return AMMSB_ERR_ARG;
     }
-    HIP_TRY(launch_loglik(a, c->prec, true, s, c->ll_pairs[1], c->ll_prefetch[1]));
+    HIP_TRY(launch_loglik(a, c->prec, true, s, c->ll_pairs[1], c->ll_prefetch[1], false));
     if (grad_theta)
       HIP_TRY(launch_theta_grad(c->d_ll_scratch, blocks, c->K, a.theta, gout, grad_theta, s));
   }

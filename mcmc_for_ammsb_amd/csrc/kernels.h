@@ -286,8 +286,12 @@ struct LogLikArgs {
 // pairs/prefetch: launch shape of the K-chunked kernel (pairs per LDS
 // exchange, next pairs' loads issued before the exchange); ignored when one
 // wavefront holds a row
+// rows: the forward pass of rows wider than 2 KB runs whole rows per
+// wavefront against the source row in LDS (k_loglik_rows) instead of the
+// K-chunked kernel
 hipError_t launch_loglik(const LogLikArgs& a, int precision, bool backward, hipStream_t s,
-                         int pairs, int prefetch);
+                         int pairs, int prefetch, bool rows);
+constexpr int kLlRowWaves = 4;  // wavefronts per workgroup of k_loglik_rows
 bool loglik_kc_shape_supported(int pairs, int prefetch);
 // Wavefronts per source of the K-chunked log-likelihood kernel used for rows
 // wider than 2 KB, 0 where one wavefront per row is used.

@@ -25,6 +25,7 @@
 // row with fully coalesced 16-byte loads.  Small K packs 64/LPR rows into
 // one wavefront.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <limits.h>
 #include <math.h>
 
@@ -2115,6 +2116,93 @@ __global__ __launch_bounds__(kLlMaxWaves * 64) void k_loglik_kc(LogLikArgs a) {
   }
 }
 
+// Forward log-likelihood for rows wider than one wavefront's 2 KB, whole
+// rows per wavefront: the source's normalised pi row and the link factors
+// live in LDS, each wavefront streams whole neighbour rows (NCH 16-byte
+// vectors per lane), so the wavefronts of a source never wait for one
+// another.  Pair p of a source goes to wavefront (p - p0) mod 4; each
+// wavefront's log-sum runs in pair order and the four are added in
+// wavefront order.
+template <typename F, int NCH>
+__global__ __launch_bounds__(kLlRowWaves * 64) void k_loglik_rows(LogLikArgs a) {
+  constexpr int V = VT<F>::V;
+  constexpr int W = kLlRowWaves;
+  constexpr int KP = NCH * 64 * V;
+  typedef typename VT<F>::type T;
+  __shared__ __align__(16) F s_pi[KP];
+  __shared__ __align__(16) F s_f1[KP];
+  __shared__ double s_red[W];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int K = a.K;
+  const F* pi = (const F*)a.pi;
+  const F eps = (F)a.epsilon;
+  const F e1 = eps, e0 = (F)1 - eps;
+  for (int k = tid; k < KP; k += W * 64) {
+    double bk = 0.0;
+    if (k < K) bk = a.theta[2 * k + 1] / (a.theta[2 * k] + a.theta[2 * k + 1]);
+    s_f1[k] = k < K ? (F)bk - eps : (F)0;
+  }
+  for (int64_t s = blockIdx.x; s < a.M; s += gridDim.x) {
+    const int32_t node = a.nodes[s];
+    const F* phis = (const F*)a.phi + s * a.phi_ld;
+    double part = 0.0;
+    for (int k = tid; k < KP; k += W * 64) {
+      const F v = k < K ? phis[k] : (F)0;
+      s_pi[k] = v;
+      part += (double)v;
+    }
+    part = group_sum<64>(part);
+    if (lane == 0) s_red[w] = part;
+    __syncthreads();
+    double sp = 0.0;
+    for (int q = 0; q < W; ++q) sp += s_red[q];  // sum(phi_s), fixed order
+    for (int k = tid; k < KP; k += W * 64) s_pi[k] = (F)((double)s_pi[k] / sp);
+    __syncthreads();
+    const int64_t p0 = min(max(a.pair_ptr[s], (int64_t)0), a.P);
+    const int64_t p1 = min(max(a.pair_ptr[s + 1], p0), a.P);
+    double ll = 0.0;
+    for (int64_t p = p0 + w; p < p1; p += W) {
+      const int32_t b = __builtin_amdgcn_readfirstlane(a.pair_nbr[p]);
+      const int y = __builtin_amdgcn_readfirstlane(a.pair_y[p] ? 1 : 0);
+      const bool bad = b < 0 || b >= a.N;
+      if (bad && lane == 0) atomicOr(a.error, ERR_BAD_INPUT);
+      const bool ok = !bad && b != node;  // mcmc_sampler_stochastic.cc:275-277
+      const F* row = pi + (int64_t)(ok ? b : 0) * a.ld;
+      F r[NCH * V];
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const int k0 = (c * 64 + lane) * V;
+        load_vec_nt<F>(row + (k0 < K ? k0 : 0), r + c * V);
+      }
+      F z = (F)0;
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const int k0 = (c * 64 + lane) * V;
+        const T pv = *reinterpret_cast<const T*>(s_pi + k0);
+        const T fv = *reinterpret_cast<const T*>(s_f1 + k0);
+        const F* pp = reinterpret_cast<const F*>(&pv);
+        const F* ff = reinterpret_cast<const F*>(&fv);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const F x = y ? fma(r[c * V + v], ff[v], e1) : fma(-r[c * V + v], ff[v], e0);
+          z = fma(pp[v], x, z);  // pi is 0 past K
+        }
+      }
+      z = group_sum<64>(z);
+      if (ok) ll += log((double)z);
+    }
+    if (lane == 0) s_red[w] = ll;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int q = 0; q < W; ++q) t += s_red[q];
+      a.per_source[s] = t;
+    }
+    __syncthreads();  // s_pi and s_red are rewritten by the next source
+  }
+}
+
 __global__ __launch_bounds__(256) void k_sum(const double* x, int64_t n, double* out) {
   __shared__ double s[256];
   double v = 0.0;
@@ -2161,15 +2249,41 @@ int loglik_kc_waves(int32_t K, int precision) {
   return w <= kLlMaxWaves ? w : 0;
 }
 
+int loglik_rows_chunks(int32_t K, int precision) {
+  const int V = precision == 0 ? 2 : 4;
+  const int vecs = (K + V - 1) / V;
+  if (vecs <= 2 * 64) return 0;  // one wavefront holds the row (k_loglik)
+  for (int n : {4, 8, 16})
+    if (vecs <= n * 64) return n;
+  return 0;
+}
+
 bool loglik_kc_shape_supported(int pairs, int prefetch) {
   return (pairs == 2 || pairs == 4 || pairs == 8) && (prefetch == 0 || prefetch == 1);
 }
 
 hipError_t launch_loglik(const LogLikArgs& a, int precision, bool backward, hipStream_t s,
-                         int pairs, int prefetch) {
+                         int pairs, int prefetch, bool rows) {
   int lpr, ch;
   if (!row_config(a.K, precision, &lpr, &ch)) return hipErrorInvalidValue;
   if (a.M <= 0) return hipSuccess;
+  const int nch = loglik_rows_chunks(a.K, precision);
+  if (rows && !backward && nch > 0) {
+    const dim3 grid((unsigned)std::min<int64_t>(a.M, 2048)), block(kLlRowWaves * 64);
+#define RW(NC)                                                                   \
+  if (nch == NC) {                                                               \
+    if (precision == 0)                                                          \
+      hipLaunchKernelGGL((k_loglik_rows<double, NC>), grid, block, 0, s, a);     \
+    else                                                                         \
+      hipLaunchKernelGGL((k_loglik_rows<float, NC>), grid, block, 0, s, a);      \
+    return hipGetLastError();                                                    \
+  }
+    RW(4)
+    RW(8)
+    RW(16)
+#undef RW
+    return hipErrorInvalidValue;
+  }
   const int waves = loglik_kc_waves(a.K, precision);
   if (waves > 0) {
     const dim3 grid(a.blocks), block(waves * 64);

@@ -165,13 +165,19 @@ def test_bad_neighbour_id_is_an_error(setup):
 
 def test_kchunk_launch_shapes_agree(setup):
     """Every (ll_pairs, ll_prefetch) shape of the K-chunked kernels sums in
-    the same order, so values and gradients are bit-identical."""
+    the same order, so values and gradients are bit-identical; the forward
+    with whole rows per wavefront (the default for wide rows) agrees with the
+    K-chunked forward to 1e-12."""
     g, s, pi, theta, nodes, ptr, nbr, y = setup
     if theta.shape[0] < 600:
         pytest.skip("one wavefront per row at this K")
     for prec, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
         smp = _sampler(g, s, pi, theta, prec)
         phi_t, th_t, n_t, p_t, b_t, y_t = _inputs(pi, theta, nodes, ptr, nbr, y, dt)
+        assert smp.get_tuning()["ll_fwd_rows"] == 1
+        rows = M.autograd.loglik_per_source(smp, phi_t.detach(), th_t.detach(), n_t, p_t, b_t,
+                                            y_t).cpu().numpy()
+        smp.set_tuning(ll_fwd_rows=0)
         ref = None
         for pairs, pf in ((4, 1), (2, 0), (2, 1), (4, 0), (8, 0), (8, 1)):
             smp.set_tuning(ll_fwd_pairs=pairs, ll_fwd_prefetch=pf, ll_bwd_pairs=pairs,
@@ -181,6 +187,8 @@ def test_kchunk_launch_shapes_agree(setup):
             got = (per.cpu().numpy(), gp.cpu().numpy(), gt.cpu().numpy())
             if ref is None:
                 ref = got
+                tol = 1e-12 if prec == "fp64" else 1e-5
+                np.testing.assert_allclose(rows, got[0], rtol=tol, atol=0)
             else:
                 for r, o in zip(ref, got):
                     np.testing.assert_array_equal(o, r, err_msg=f"{prec} shape ({pairs},{pf})")
