@@ -246,6 +246,8 @@ def run_diff(args):
         smp.init(42)
         if args.ll_fwd_rows >= 0:
             smp.set_tuning(ll_fwd_rows=args.ll_fwd_rows)
+        if args.ll_bwd_rows >= 0:
+            smp.set_tuning(ll_bwd_rows=args.ll_bwd_rows)
         rows = smp.get_rows(nodes)
         theta = np.empty((K, 2))
         _lib.check(_lib.lib.ammsb_get_state(smp._h, None, _lib.ptr(theta, ctypes.c_double), None))
@@ -458,6 +460,8 @@ def main():
     ap.add_argument("--diff-reps", type=int, default=10)
     ap.add_argument("--ll-fwd-rows", type=int, default=-1,
                     help="diff mode: 1 whole rows per wavefront, 0 the K-chunked forward")
+    ap.add_argument("--ll-bwd-rows", type=int, default=-1,
+                    help="diff mode: 1 whole rows per wavefront, 0 the K-chunked adjoint")
     ap.add_argument("--ttt-steps", type=int, default=3000)
     ap.add_argument("--ttt-interval", type=int, default=100)
     args = ap.parse_args()

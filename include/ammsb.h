@@ -297,9 +297,10 @@ int ammsb_get_totals(ammsb_ctx *ctx, int64_t *nodes_updated,
  *                wider than 2 KB): 2, 4, 8
  *   "ll_fwd_prefetch", "ll_bwd_prefetch"  1: the next pairs' rows are
  *                loaded before the exchange
- *   "ll_fwd_rows" 1 (default): the forward pass of rows wider than 2 KB
- *                streams whole rows per wavefront against the source row
- *                held in LDS; 0: the K-chunked kernel above
+ *   "ll_fwd_rows", "ll_bwd_rows"  1: the forward / adjoint pass of rows
+ *                wider than 2 KB streams whole rows per wavefront against
+ *                the source row held in LDS; 0: the K-chunked kernels above
+ *                (defaults: forward 1; adjoint 1 for FP64, 0 for FP32)
  *   "phi_split"  1: update_phi as a gather kernel + a phi-step kernel
  *                instead of the fused step + scatter (default 0)
  *   "async_ring" 1: the minibatch/neighbour samplers draw the next 16

@@ -121,6 +121,7 @@ struct ammsb_ctx {
   // K-chunked log-likelihood launch shapes [0] forward, [1] adjoint
   int32_t ll_pairs[2] = {0, 0}, ll_prefetch[2] = {0, 0};
   int32_t ll_fwd_rows = 1;  // forward of wide rows: whole rows per wavefront (else K-chunked)
+  int32_t ll_bwd_rows = 1;  // adjoint of wide rows: the same (default FP64 only)
   double* d_ll_scratch = nullptr;   // differentiation: per-source values / dbeta partials
   size_t ll_scratch_bytes = 0;
   int2* d_nbr_gtable = nullptr;
@@ -601,6 +602,9 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
   c->ll_prefetch[0] = 0;
   c->ll_pairs[1] = c->prec == AMMSB_FP64 ? 4 : 2;
   c->ll_prefetch[1] = c->prec == AMMSB_FP64 ? 0 : 1;
+  // whole-row adjoint: faster for FP64 (C4: 8.6 -> 7.4 ms), slower for FP32
+  // (4.2 -> 4.8 ms: its LDS traffic per byte of row doubles)
+  c->ll_bwd_rows = c->prec == AMMSB_FP64 ? 1 : 0;
   int lpr, ch;
   if (!row_config(c->K, c->prec, &lpr, &ch)) {
     set_error("ammsb_create: K too large for the device kernels (max 2048 FP64 / 4096 FP32)");
@@ -1391,9 +1395,9 @@ int ammsb_set_tuning(ammsb_ctx* c, const char* name, int64_t value) {
     c->ring_valid = false;
     return AMMSB_OK;
   }
-  if (n == "ll_fwd_rows") {
+  if (n == "ll_fwd_rows" || n == "ll_bwd_rows") {
     if (value != 0 && value != 1) return AMMSB_ERR_ARG;
-    c->ll_fwd_rows = (int32_t)value;
+    (n == "ll_fwd_rows" ? c->ll_fwd_rows : c->ll_bwd_rows) = (int32_t)value;
     return AMMSB_OK;
   }
   if (n == "ll_fwd_pairs" || n == "ll_fwd_prefetch" || n == "ll_bwd_pairs" ||
@@ -1451,6 +1455,8 @@ int ammsb_get_tuning(ammsb_ctx* c, const char* name, int64_t* value) {
     *value = c->ll_prefetch[1];
   else if (n == "ll_fwd_rows")
     *value = c->ll_fwd_rows;
+  else if (n == "ll_bwd_rows")
+    *value = c->ll_bwd_rows;
   else if (n == "sampler_lds")  // read-only: bit 0 minibatch table, bit 1 neighbour tables in LDS
     *value = (c->mbtable_in_lds ? 1 : 0) | (c->nbr_use_lds ? 2 : 0);
   else {
@@ -1499,10 +1505,16 @@ int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi
   HIP_TRY(hipSetDevice(c->opt.device));
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   if (stream) HIP_TRY(hipStreamSynchronize(c->stream));  // resident state settled
-  const int32_t blocks = (int32_t)std::max<int64_t>(1, std::min<int64_t>(M, 1024));
-  // scratch: per-source values (fwd) or [blocks][K] dbeta partials (bwd), plus
+  // the whole-row adjoint holds 96 KB of LDS per workgroup (one per CU) and
+  // writes a beta partial row per wavefront
+  const bool rows_bwd = bwd && c->ll_bwd_rows && loglik_rows_chunks(c->K, c->prec) > 0;
+  const int32_t blocks =
+      (int32_t)std::max<int64_t>(1, std::min<int64_t>(M, rows_bwd ? 256 : 1024));
+  const int64_t prows = rows_bwd ? (int64_t)blocks * kLlRowWaves : blocks;
+  // scratch: per-source values (fwd) or [prows][K] dbeta partials (bwd), plus
   // one double for an absent total and the error word
-  const size_t need = sizeof(double) * (size_t)(std::max<int64_t>(M, (int64_t)blocks * c->K) + 2);
+  const int64_t body = std::max<int64_t>(M, prows * c->K);
+  const size_t need = sizeof(double) * (size_t)(body + 2);
   if (need > c->ll_scratch_bytes) {
     if (c->d_ll_scratch) HIP_TRY(hipFree(c->d_ll_scratch));
     c->d_ll_scratch = nullptr;
@@ -1510,7 +1522,7 @@ int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi
     HIP_TRY(hipMalloc((void**)&c->d_ll_scratch, need));
     c->ll_scratch_bytes = need;
   }
-  int32_t* d_err = (int32_t*)(c->d_ll_scratch + std::max<int64_t>(M, (int64_t)blocks * c->K) + 1);
+  int32_t* d_err = (int32_t*)(c->d_ll_scratch + body + 1);
   HIP_TRY(launch_fill_u32((uint32_t*)d_err, 0u, 1, s));  // a kernel, not a memset node
   LogLikArgs a{};
   a.pi = c->d_pi;
@@ -1536,16 +1548,17 @@ int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi
   if (!bwd) {
     HIP_TRY(launch_loglik(a, c->prec, false, s, c->ll_pairs[0], c->ll_prefetch[0],
                           c->ll_fwd_rows != 0));
-    double* out = total ? total : c->d_ll_scratch + std::max<int64_t>(M, (int64_t)blocks * c->K);
+    double* out = total ? total : c->d_ll_scratch + body;
     HIP_TRY(launch_sum(a.per_source, M, out, s));
   } else {
     if (!grad_phi) {
       set_error("ammsb_loglik_bwd: grad_phi is required");
       return AMMSB_ERR_ARG;
     }
-    HIP_TRY(launch_loglik(a, c->prec, true, s, c->ll_pairs[1], c->ll_prefetch[1], false));
+    HIP_TRY(launch_loglik(a, c->prec, true, s, c->ll_pairs[1], c->ll_prefetch[1], rows_bwd));
     if (grad_theta)
-      HIP_TRY(launch_theta_grad(c->d_ll_scratch, blocks, c->K, a.theta, gout, grad_theta, s));
+      HIP_TRY(launch_theta_grad(c->d_ll_scratch, (int32_t)prows, c->K, a.theta, gout, grad_theta,
+                                s));
   }
   int32_t herr = 0;
   HIP_TRY(hipMemcpyAsync(&herr, d_err, sizeof(int32_t), hipMemcpyDeviceToHost, s));

@@ -2203,6 +2203,136 @@ __global__ __launch_bounds__(kLlRowWaves * 64) void k_loglik_rows(LogLikArgs a) 
   }
 }
 
+// Adjoint of k_loglik_rows: each wavefront streams whole neighbour rows,
+// takes Z of a pair from the whole row, and adds x/Z - 1 into its own
+// [K] array in LDS (no cross-wavefront exchange per pair) and the beta
+// terms into its registers.  At the end of a source the four arrays are
+// added in wavefront order into grad_phi; each wavefront writes its beta
+// partial row (partial rows blockIdx*4 + wavefront).  Dynamic LDS: the
+// source row, the link factors and the four arrays, (2 + 4) * NCH*64*V
+// Floats.
+template <typename F, int NCH>
+__global__ __launch_bounds__(kLlRowWaves * 64) void k_loglik_rows_bwd(LogLikArgs a) {
+  constexpr int V = VT<F>::V;
+  constexpr int W = kLlRowWaves;
+  constexpr int KP = NCH * 64 * V;
+  typedef typename VT<F>::type T;
+  extern __shared__ __align__(16) unsigned char s_dynb[];
+  F* s_pi = reinterpret_cast<F*>(s_dynb);
+  F* s_f1 = s_pi + KP;
+  F* s_acc = s_f1 + KP;  // [W][KP]
+  __shared__ double s_red[W];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int K = a.K;
+  const F* pi = (const F*)a.pi;
+  const F eps = (F)a.epsilon;
+  const F e1 = eps, e0 = (F)1 - eps;
+  const double go = a.gout[0];
+  F* my_acc = s_acc + w * KP;
+  for (int k = tid; k < KP; k += W * 64) {
+    double bk = 0.0;
+    if (k < K) bk = a.theta[2 * k + 1] / (a.theta[2 * k] + a.theta[2 * k + 1]);
+    s_f1[k] = k < K ? (F)bk - eps : (F)0;
+  }
+  double db[NCH * V];
+#pragma unroll
+  for (int j = 0; j < NCH * V; ++j) db[j] = 0.0;
+  for (int64_t s = blockIdx.x; s < a.M; s += gridDim.x) {
+    const int32_t node = a.nodes[s];
+    const F* phis = (const F*)a.phi + s * a.phi_ld;
+    double part = 0.0;
+    for (int k = tid; k < KP; k += W * 64) {
+      const F v = k < K ? phis[k] : (F)0;
+      s_pi[k] = v;
+      part += (double)v;
+    }
+    part = group_sum<64>(part);
+    if (lane == 0) s_red[w] = part;
+    __syncthreads();
+    double sp = 0.0;
+    for (int q = 0; q < W; ++q) sp += s_red[q];  // sum(phi_s), fixed order
+    for (int k = tid; k < KP; k += W * 64) s_pi[k] = (F)((double)s_pi[k] / sp);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      T zero;
+      F* zp = reinterpret_cast<F*>(&zero);
+#pragma unroll
+      for (int v = 0; v < V; ++v) zp[v] = (F)0;
+      *reinterpret_cast<T*>(my_acc + (c * 64 + lane) * V) = zero;
+    }
+    __syncthreads();
+    const int64_t p0 = min(max(a.pair_ptr[s], (int64_t)0), a.P);
+    const int64_t p1 = min(max(a.pair_ptr[s + 1], p0), a.P);
+    for (int64_t p = p0 + w; p < p1; p += W) {
+      const int32_t b = __builtin_amdgcn_readfirstlane(a.pair_nbr[p]);
+      const int y = __builtin_amdgcn_readfirstlane(a.pair_y[p] ? 1 : 0);
+      const bool bad = b < 0 || b >= a.N;
+      if (bad && lane == 0) atomicOr(a.error, ERR_BAD_INPUT);
+      const bool ok = !bad && b != node;  // mcmc_sampler_stochastic.cc:275-277
+      const F* row = pi + (int64_t)(ok ? b : 0) * a.ld;
+      F r[NCH * V];
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const int k0 = (c * 64 + lane) * V;
+        load_vec_nt<F>(row + (k0 < K ? k0 : 0), r + c * V);
+      }
+      F z = (F)0;
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const int k0 = (c * 64 + lane) * V;
+        const T pv = *reinterpret_cast<const T*>(s_pi + k0);
+        const T fv = *reinterpret_cast<const T*>(s_f1 + k0);
+        const F* pp = reinterpret_cast<const F*>(&pv);
+        const F* ff = reinterpret_cast<const F*>(&fv);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const F x = y ? fma(r[c * V + v], ff[v], e1) : fma(-r[c * V + v], ff[v], e0);
+          z = fma(pp[v], x, z);  // pi is 0 past K
+        }
+      }
+      z = group_sum<64>(z);
+      if (!ok) continue;  // uniform
+      const F rz = recip(z);
+      const double sg = y ? 1.0 : -1.0;
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const int k0 = (c * 64 + lane) * V;
+        const T pv = *reinterpret_cast<const T*>(s_pi + k0);
+        const T fv = *reinterpret_cast<const T*>(s_f1 + k0);
+        T av = *reinterpret_cast<const T*>(my_acc + k0);
+        const F* pp = reinterpret_cast<const F*>(&pv);
+        const F* ff = reinterpret_cast<const F*>(&fv);
+        F* aa = reinterpret_cast<F*>(&av);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const F rv = r[c * V + v];
+          const F x = y ? fma(rv, ff[v], e1) : fma(-rv, ff[v], e0);
+          aa[v] += fma(x, rz, (F)-1);
+          db[c * V + v] += sg * (double)(pp[v] * rv * rz);
+        }
+        *reinterpret_cast<T*>(my_acc + k0) = av;
+      }
+    }
+    __syncthreads();
+    F* gp = (F*)a.grad_phi + s * a.phi_ld;
+    for (int k = tid; k < K; k += W * 64) {
+      F t = s_acc[k];
+      for (int q = 1; q < W; ++q) t += s_acc[q * KP + k];
+      gp[k] = (F)(go * (double)t / sp);
+    }
+    __syncthreads();  // s_pi, s_acc and s_red are rewritten by the next source
+  }
+  double* out = a.dbeta_partials + ((int64_t)blockIdx.x * W + w) * K;
+#pragma unroll
+  for (int c = 0; c < NCH; ++c)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int k = (c * 64 + lane) * V + v;
+      if (k < K) out[k] = db[c * V + v];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_sum(const double* x, int64_t n, double* out) {
   __shared__ double s[256];
   double v = 0.0;
@@ -2268,6 +2398,34 @@ hipError_t launch_loglik(const LogLikArgs& a, int precision, bool backward, hipS
   if (!row_config(a.K, precision, &lpr, &ch)) return hipErrorInvalidValue;
   if (a.M <= 0) return hipSuccess;
   const int nch = loglik_rows_chunks(a.K, precision);
+  if (rows && backward && nch > 0) {
+    // a.blocks workgroups: a.blocks * kLlRowWaves beta partial rows
+    const int fs = precision == 0 ? 8 : 4;
+    const size_t lds = (size_t)(2 + kLlRowWaves) * nch * 64 * (16 / fs) * fs;
+    const dim3 grid(a.blocks), block(kLlRowWaves * 64);
+#define RB(NC)                                                                          \
+  if (nch == NC) {                                                                      \
+    if (precision == 0) {                                                               \
+      hipError_t e = hipFuncSetAttribute((const void*)k_loglik_rows_bwd<double, NC>,   \
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,    \
+                                         (int)lds);                                     \
+      if (e != hipSuccess) return e;                                                    \
+      hipLaunchKernelGGL((k_loglik_rows_bwd<double, NC>), grid, block, lds, s, a);      \
+    } else {                                                                            \
+      hipError_t e = hipFuncSetAttribute((const void*)k_loglik_rows_bwd<float, NC>,    \
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,    \
+                                         (int)lds);                                     \
+      if (e != hipSuccess) return e;                                                    \
+      hipLaunchKernelGGL((k_loglik_rows_bwd<float, NC>), grid, block, lds, s, a);       \
+    }                                                                                   \
+    return hipGetLastError();                                                           \
+  }
+    RB(4)
+    RB(8)
+    RB(16)
+#undef RB
+    return hipErrorInvalidValue;
+  }
   if (rows && !backward && nch > 0) {
     const dim3 grid((unsigned)std::min<int64_t>(a.M, 2048)), block(kLlRowWaves * 64);
 #define RW(NC)                                                                   \

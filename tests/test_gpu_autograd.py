@@ -175,9 +175,12 @@ def test_kchunk_launch_shapes_agree(setup):
         smp = _sampler(g, s, pi, theta, prec)
         phi_t, th_t, n_t, p_t, b_t, y_t = _inputs(pi, theta, nodes, ptr, nbr, y, dt)
         assert smp.get_tuning()["ll_fwd_rows"] == 1
+        smp.set_tuning(ll_bwd_rows=1)
         rows = M.autograd.loglik_per_source(smp, phi_t.detach(), th_t.detach(), n_t, p_t, b_t,
                                             y_t).cpu().numpy()
-        smp.set_tuning(ll_fwd_rows=0)
+        rgp, rgt = M.autograd.loglik_grad(smp, phi_t.detach(), th_t.detach(), n_t, p_t, b_t, y_t)
+        rows_g = (rgp.cpu().numpy(), rgt.cpu().numpy())
+        smp.set_tuning(ll_fwd_rows=0, ll_bwd_rows=0)
         ref = None
         for pairs, pf in ((4, 1), (2, 0), (2, 1), (4, 0), (8, 0), (8, 1)):
             smp.set_tuning(ll_fwd_pairs=pairs, ll_fwd_prefetch=pf, ll_bwd_pairs=pairs,
@@ -189,6 +192,8 @@ def test_kchunk_launch_shapes_agree(setup):
                 ref = got
                 tol = 1e-12 if prec == "fp64" else 1e-5
                 np.testing.assert_allclose(rows, got[0], rtol=tol, atol=0)
+                for r, o in zip(rows_g, got[1:]):
+                    np.testing.assert_allclose(r, o, rtol=tol * 10, atol=tol * np.abs(o).max())
             else:
                 for r, o in zip(ref, got):
                     np.testing.assert_array_equal(o, r, err_msg=f"{prec} shape ({pairs},{pf})")
