@@ -212,6 +212,35 @@ def test_samplers_bit_exact(strategy, max_source):
         assert flags == {0, 1}
 
 
+@pytest.mark.parametrize("strategy,max_source", [("stratified-random-node", 4),
+                                                 ("random-edge", 1)])
+def test_tiny_graph(strategy, max_source):
+    """N=64 (185 edges): the node list fills most of the graph, the
+    neighbour draws 9 of 63 nodes.  Draws bit-exact against the oracle, then
+    40 iterations keep every row normalised and the perplexity finite."""
+    print(f"tiny graph {strategy}", flush=True)
+    N, K, n, m = 64, 4, 8, 20
+    strat = {"stratified-random-node": 0, "random-edge": 3}[strategy]
+    g, s, smp, orc = make_pair("community", N, K, n=n, m=m, strategy=strategy,
+                               max_source=max_source, seed=5, deg=6.0)
+    mm = smp.info["mini_batch_size"]
+    for step in range(1, 9):
+        d = smp.debug_sample(step)
+        r = orc.sample_minibatch(strat, mm, max_source, 5, step)
+        assert d["flag"] == r["flag"], step
+        assert np.array_equal(d["edges"], r["edges"]), step
+        assert np.array_equal(np.sort(d["nodes"]), np.unique(r["edges"])), step
+        for q, v in enumerate(d["nodes"]):
+            assert np.array_equal(d["nbrs"][q], orc.sample_neighbors(int(v), n + 1, 5, step))
+    smp.init(3)
+    smp.step(40)
+    pi, theta, beta = smp.get_state()
+    assert np.all(np.isfinite(pi)) and np.all(pi[:, K] > 0)
+    assert np.allclose(pi[:, :K].sum(1), 1.0, atol=1e-12)
+    assert np.all((beta > 0) & (beta < 1))
+    assert np.isfinite(smp.perplexity())
+
+
 def test_trajectory_c1_within_one_percent():
     """Config C1 (N=5000, K=32, n=32, a=0.01, b=1024, c=0.55) from the same
     initial state and Philox seed: held-out perplexity within 1% at every
