@@ -241,6 +241,32 @@ def test_tiny_graph(strategy, max_source):
     assert np.isfinite(smp.perplexity())
 
 
+def test_run_stops_at_convergence():
+    """run() stops once the relative change of the last two held-out
+    perplexities is at most convergence_threshold (learner.cc:130-136): with
+    a threshold of 1 the loop ends after the second check, one iteration
+    past it; with 0 it runs to max_iteration."""
+    print("convergence stop", flush=True)
+    N, K = 2000, 16
+    g, s = graph("community", N, K)
+    opts = M.Options(K=K, num_node_sample=8, interval=10, max_iteration=200,
+                     convergence_threshold=1.0)
+    smp = M.MCMCSamplerStochastic(opts, g, s)
+    smp.init(2)
+    ck = smp.run(print_log=False)
+    assert [t for t, _ in ck] == [1, 11]
+    assert smp.info["step_count"] == 12
+    smp.close()
+    opts = M.Options(K=K, num_node_sample=8, interval=10, max_iteration=45,
+                     convergence_threshold=0.0)
+    smp = M.MCMCSamplerStochastic(opts, g, s)
+    smp.init(2)
+    ck = smp.run(print_log=False)
+    assert [t for t, _ in ck] == [1, 11, 21, 31, 41]
+    assert smp.info["step_count"] == 45
+    smp.close()
+
+
 def test_trajectory_c1_within_one_percent():
     """Config C1 (N=5000, K=32, n=32, a=0.01, b=1024, c=0.55) from the same
     initial state and Philox seed: held-out perplexity within 1% at every
