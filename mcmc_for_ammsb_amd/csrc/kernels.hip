@@ -1301,19 +1301,25 @@ __global__ __launch_bounds__(WAVES * 64) void k_ppx(PpxArgs a) {
     }
   double ll1 = 0.0, ll0 = 0.0, n1 = 0.0, n0 = 0.0;
   const double cnt = a.avg_count;
-  // Two held-out edges per row group in flight: ids, labels and running
-  // averages first, then all four rows, then the reductions.
+  // Each workgroup takes a contiguous range of the held-out list (sorted by
+  // the first endpoint), so the pairs that share row a run on one CU and
+  // find it in that XCD's L2.  Two held-out edges per row group in flight:
+  // ids, labels and running averages first, then all four rows, then the
+  // reductions.
   constexpr int UE = CH <= 4 ? 2 : 1;  // big rows: one edge at a time (registers)
-  const int64_t gstride = (int64_t)a.blocks * WAVES * RPW;
-  for (int64_t e0 = ((int64_t)blockIdx.x * WAVES + w) * RPW + g; e0 < a.H; e0 += UE * gstride) {
+  constexpr int64_t kPer = (int64_t)WAVES * RPW;
+  const int64_t chunk = ((a.H + a.blocks - 1) / a.blocks + kPer - 1) / kPer * kPer;
+  const int64_t e_lo = (int64_t)blockIdx.x * chunk;
+  const int64_t e_hi = min(a.H, e_lo + chunk);
+  for (int64_t e0 = e_lo + w * RPW + g; e0 < e_hi; e0 += UE * kPer) {
     int32_t ea[UE], eb[UE];
     int yv[UE];
     bool val[UE];
     double rold[UE];
 #pragma unroll
     for (int u = 0; u < UE; ++u) {
-      const int64_t e = e0 + u * gstride;
-      val[u] = e < a.H;
+      const int64_t e = e0 + u * kPer;
+      val[u] = e < e_hi;
       const int64_t ee = val[u] ? e : e0;
       ea[u] = a.edges[2 * ee];
       eb[u] = a.edges[2 * ee + 1];
@@ -1351,7 +1357,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_ppx(PpxArgs a) {
       if (s < (F)1e-30) s = (F)1e-30;
       if (lg == 0) {
         const double r = (rold[u] * (cnt - 1.0) + (double)s) / cnt;
-        a.R[e0 + u * gstride] = r;
+        a.R[e0 + u * kPer] = r;
         const double lr = log(r);
         if (y) {
           ll1 += lr;
