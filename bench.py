@@ -466,8 +466,8 @@ def main():
     ap.add_argument("--ttt-interval", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
-        log("bench.py measures one GPU: this build has no multi-GPU path "
-            "(every box has a single MI355X); run with --gpus 1")
+        log("bench.py measures one GPU: the library has no multi-GPU path "
+            "(one MI355X per process); run with --gpus 1")
         return 2
     rank = int(os.environ.get("RANK", "0"))
     if args.mode == "ttt":

@@ -602,8 +602,9 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
   c->ll_prefetch[0] = 0;
   c->ll_pairs[1] = c->prec == AMMSB_FP64 ? 4 : 2;
   c->ll_prefetch[1] = c->prec == AMMSB_FP64 ? 0 : 1;
-  // whole-row adjoint: faster for FP64 (C4: 8.6 -> 7.4 ms), slower for FP32
-  // (4.2 -> 4.8 ms: its LDS traffic per byte of row doubles)
+  // whole-row adjoint: faster for FP64 (N=1M, K=2048, 2.5M pairs: 8.6 -> 7.4
+  // ms), slower for FP32 (4.2 -> 4.8 ms: its LDS traffic per byte of row
+  // doubles)
   c->ll_bwd_rows = c->prec == AMMSB_FP64 ? 1 : 0;
   int lpr, ch;
   if (!row_config(c->K, c->prec, &lpr, &ch)) {
@@ -845,9 +846,10 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
     const NeighborArgs na = neighbor_args(c);
     HIP_TRY(hipMemcpy(c->d_sargs1, &sa, sizeof sa, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->d_nargs1, &na, sizeof na, hipMemcpyHostToDevice));
-    // off by default: measured on one MI355X, the overlap is a wash (C2 FP64
-    // 0.261 vs 0.262 ms per step, C3 FP32 0.654 vs 0.650) because the
-    // samplers take CUs from update_phi; set_tuning("async_ring", 1) enables it
+    // off by default: measured on one MI355X, the overlap is a wash (N=1M
+    // K=256 FP64: 0.261 vs 0.262 ms per step; N=4M K=1024 FP32: 0.654 vs
+    // 0.650) because the samplers take CUs from update_phi;
+    // set_tuning("async_ring", 1) enables it
     c->async_ring = false;
   }
   HIP_TRY(hipStreamSynchronize(c->stream));

@@ -267,11 +267,11 @@ def test_run_stops_at_convergence():
     smp.close()
 
 
-def test_trajectory_c1_within_one_percent():
-    """Config C1 (N=5000, K=32, n=32, a=0.01, b=1024, c=0.55) from the same
-    initial state and Philox seed: held-out perplexity within 1% at every
-    checkpoint of 1000 iterations."""
-    print("trajectory C1", flush=True)
+def test_trajectory_n5000_within_one_percent():
+    """The reference's CPU-sized workload (N=5000, K=32, n=32, a=0.01,
+    b=1024, c=0.55) from the same initial state and Philox seed: held-out
+    perplexity within 1% at every checkpoint of 1000 iterations."""
+    print("trajectory N=5000 K=32", flush=True)
     N, K = 5000, 32
     g, s, smp, orc = make_pair("planted", N, K, n=32, m=500, a=0.01, b=1024.0, c=0.55,
                                interval=50, seed=42)
