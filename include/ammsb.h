@@ -142,6 +142,29 @@ const char *ammsb_version(void);
 /* Fill with the reference defaults (options.h:139-212). */
 int ammsb_options_default(ammsb_options *opts);
 
+/* The input options of mcmc::Options (options.h:192-205). */
+typedef struct ammsb_input {
+  char filename[4096];   /* --mcmc.input.file, -f ("")            */
+  char input_class[64];  /* --mcmc.input.class, -c ("relativity") */
+  int32_t contiguous;    /* --mcmc.input.contiguous (0)           */
+  int32_t compressed;    /* --mcmc.input.compressed (0)           */
+} ammsb_input;
+
+/* Replaces mcmc::Options::Parse (options.h:236-255, boost program_options
+ * with allow_unregistered): argv[0 .. argc) (no program name) -> defaults
+ * overridden by --mcmc.* flags (--name value or --name=value), the short
+ * aliases -K -m -n -x -i -I -h -f -c (-K32 or -K 32), the input switches,
+ * --verbose, --mcmc.precision fp64|fp32, --mcmc.device, and --config FILE
+ * ("name = value" lines, [section] prefixes, '#' comments; the command line
+ * wins).  Integers parse as the reference's parse<T> (options.h:39-57,
+ * options.cc:5-67: 0x hex, leading-0 octal, a k/kb m/mb g/gb suffix for
+ * 2^10/2^20/2^30).  Unrecognised arguments are skipped and counted in
+ * *n_unrecognised (may be null).  a, alpha, interval, ... keep 0 where not
+ * given; ammsb_create derives them.  A malformed value is AMMSB_ERR_ARG
+ * (the reference exits 33 on an option error, main.cc:83-86). */
+int ammsb_options_parse(int32_t argc, const char *const *argv, ammsb_options *opts,
+                        ammsb_input *input, int32_t *n_unrecognised);
+
 /* ---- host-side graph tools (no GPU needed) ----------------------------- */
 /* Synthetic generators (see ammsb_gen_params).  The library allocates
  * *row_ptr (N+1) and *cols (2E); release both with ammsb_free(). */

@@ -51,6 +51,11 @@ class Options(ctypes.Structure):
     ]
 
 
+class Input(ctypes.Structure):
+    _fields_ = [("filename", ctypes.c_char * 4096), ("input_class", ctypes.c_char * 64),
+                ("contiguous", c_i32), ("compressed", c_i32)]
+
+
 class Graph(ctypes.Structure):
     _fields_ = [("N", c_i32), ("num_edges", c_i64), ("row_ptr", P_i64), ("cols", P_i32)]
 
@@ -137,6 +142,9 @@ def _load():
         "ammsb_last_error": (ctypes.c_char_p, []),
         "ammsb_version": (ctypes.c_char_p, []),
         "ammsb_options_default": (c_i32, [ctypes.POINTER(Options)]),
+        "ammsb_options_parse": (c_i32, [c_i32, ctypes.POINTER(ctypes.c_char_p),
+                                        ctypes.POINTER(Options), ctypes.POINTER(Input),
+                                        ctypes.POINTER(c_i32)]),
         "ammsb_graph_generate": (c_i32, [ctypes.POINTER(GenParams), ctypes.POINTER(P_i64),
                                          ctypes.POINTER(P_i32), P_i64]),
         "ammsb_split_generate": (c_i32, [ctypes.POINTER(Graph), c_dbl, c_u64,

@@ -14,6 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libammsb.so")
+DRIVER = os.path.join(HERE, "ammsb_mcmc")  # command-line driver executable
 OBJDIR = os.path.join(HERE, "_build")
 
 ARCH = os.environ.get("AMMSB_OFFLOAD_ARCH", "gfx950")
@@ -25,7 +26,7 @@ COMMON = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas"]
 
-SOURCES = ["kernels.hip", "capi.hip", "host_graph.cpp"]
+SOURCES = ["kernels.hip", "capi.hip", "host_graph.cpp", "host_options.cpp"]
 
 
 def _compile(src):
@@ -56,6 +57,14 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stderr}")
     os.replace(tmp, OUT)
+    # the command-line driver (the reference's main), linked against the library
+    drv_tmp = DRIVER + ".tmp"
+    cmd = [CXX, "-std=c++17", "-O2", "-Wall", os.path.join(CSRC, "ammsb_main.cpp"), "-o", drv_tmp,
+           "-L" + HERE, "-lammsb", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver link failed: {' '.join(cmd)}\n{r.stderr}")
+    os.replace(drv_tmp, DRIVER)
     return OUT
 
 

@@ -5,8 +5,8 @@ Reference: c++/src/mcmc/options.h:129-315 (Boost program_options with dotted
 strategy names of c++/src/mcmc/types.h:11-60.  Defaults and the derived
 defaults (``a = b^-c``, ``alpha = 1/K``, options.h:257-262) are the same;
 unrecognised arguments are kept in ``remains`` like ``allow_unregistered``
-(options.h:244-255).  Values accept the reference's K/M/G suffixes
-(``parse_size_t``, c++/src/mcmc/options.cc:5-67).
+(options.h:244-255).  Integers parse as the reference's ``parse<T>`` (0x hex,
+leading-0 octal, k/m/g suffixes; c++/src/mcmc/options.h:39-57, options.cc:5-67).
 """
 import ctypes
 import dataclasses
@@ -22,19 +22,59 @@ STRATEGIES = {
 }
 
 
-def parse_size(text):
-    """Integer with optional k/K/m/M/g/G (powers of 1024) or kB/mB (powers of
-    1000) suffix, as options.cc:5-67."""
-    s = str(text).strip()
-    mult = 1
-    low = s.lower()
-    for suf, m in (("kb", 1000), ("mb", 1000 ** 2), ("gb", 1000 ** 3),
-                   ("k", 1024), ("m", 1024 ** 2), ("g", 1024 ** 3)):
-        if low.endswith(suf) and len(s) > len(suf):
-            mult = m
-            s = s[: -len(suf)]
+def _parse_size_t(arg):
+    """parse_size_t (options.cc:5-67): "0x" hexadecimal, a leading "0" octal
+    (its digit test accepts 8 and 9, as there), else decimal; the rest of
+    the string may be one suffix k/kb, m/mb, g/gb (any case) for 2^10,
+    2^20, 2^30."""
+    n, base, i = 0, 10, 0
+    if arg[:2] in ("0x", "0X"):
+        base, i = 16, 2
+    elif arg[:1] == "0":
+        base, i = 8, 1
+    while i < len(arg):
+        ch = arg[i]
+        if base == 16 and ch in "0123456789abcdefABCDEF":
+            n = n * 16 + int(ch, 16)
+        elif base <= 10 and ch.isdigit() and ch.isascii():
+            n = n * base + (ord(ch) - ord("0"))
+        else:
+            rest = arg[i:].lower()
+            shift = {"g": 30, "gb": 30, "m": 20, "mb": 20, "k": 10, "kb": 10}.get(rest)
+            if shift is None:
+                raise ValueError(f"Unknown characters in number: {ch!r}")
+            n <<= shift
             break
-    return int(s, 0) * mult
+        if n >= 1 << 64:
+            raise ValueError("Overflow in parse_size_t")
+        i += 1
+    if n >= 1 << 64:
+        raise ValueError("Overflow in parse_size_t")
+    return n
+
+
+def _parse_integral(text, lo, hi):
+    """mcmc::parse<T> (options.h:39-57): a leading '-' negates."""
+    s = str(text)
+    if s.startswith("-"):
+        v = -_parse_size_t(s[1:])
+        if v < lo:
+            raise ValueError("number too small for type")
+        return v
+    v = _parse_size_t(s)
+    if v > hi:
+        raise ValueError("number too large for type")
+    return v
+
+
+def parse_size(text):
+    """A size_t option (no negative values)."""
+    return _parse_integral(text, 0, (1 << 63) - 1)
+
+
+def parse_int(text):
+    """An int option."""
+    return _parse_integral(text, -(1 << 31), (1 << 31) - 1)
 
 
 @dataclasses.dataclass
@@ -85,13 +125,16 @@ class Options:
         "mcmc.stats": ("stats_print_interval", parse_size), "I": ("stats_print_interval", parse_size),
         "mcmc.num-updates": ("num_updates", parse_size),
         "mcmc.held-out-ratio": ("held_out_ratio", float), "h": ("held_out_ratio", float),
-        "mcmc.seed": ("random_seed", int),
+        "mcmc.seed": ("random_seed", parse_int),
         "mcmc.convergence": ("convergence_threshold", float),
         "mcmc.input.file": ("input_filename", str), "f": ("input_filename", str),
         "mcmc.input.class": ("input_class", str), "c": ("input_class", str),
         "mcmc.precision": ("precision", str),
-        "mcmc.device": ("device", int),
+        "mcmc.device": ("device", parse_int),
     }
+    _ALIASES = {"K": "mcmc.K", "m": "mcmc.mini-batch-size", "n": "mcmc.num-node-sample",
+                "x": "mcmc.max-iteration", "i": "mcmc.interval", "I": "mcmc.stats",
+                "h": "mcmc.held-out-ratio", "f": "mcmc.input.file", "c": "mcmc.input.class"}
     _SWITCHES = {"mcmc.input.contiguous": "input_contiguous",
                  "mcmc.input.compressed": "input_compressed", "verbose": "verbose"}
 
@@ -150,7 +193,14 @@ class Options:
     def _read_config(self, path, cmdline):
         # Boost parse_config_file: "name = value" lines, [section] prefixes;
         # command-line values win (stored first, options.h:249-252)
-        given = {t[2:].partition("=")[0] for t in cmdline if t.startswith("--")}
+        given = set()
+        for t in cmdline:
+            if t.startswith("--"):
+                given.add(t[2:].partition("=")[0])
+            elif t.startswith("-") and len(t) >= 2 and not _is_number(t):
+                long_name = self._ALIASES.get(t[1:2])
+                if long_name:
+                    given.add(long_name)
         section = ""
         with open(path) as f:
             for line in f:

@@ -267,6 +267,41 @@ def test_run_stops_at_convergence():
     smp.close()
 
 
+def test_driver_matches_python_run(tmp_path):
+    """The command-line driver (the reference's main: edge-list input,
+    split, init, run with its log line) prints the same held-out
+    perplexities as the Python front end on the same file and options."""
+    import os
+    import re
+    import subprocess
+    print("command-line driver", flush=True)
+    driver = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "mcmc_for_ammsb_amd",
+                          "ammsb_mcmc")
+    g0 = M.generate("community", N=3000, K=8, mean_degree=10.0, seed=4)
+    src = np.repeat(np.arange(g0.N), np.diff(g0.row_ptr))
+    keep = src < g0.cols
+    path = tmp_path / "graph.txt"
+    with open(path, "w") as f:
+        f.write("# synthetic\n# community graph\n# Nodes: 3000\n# FromNodeId ToNodeId\n")
+        for a, b in zip(src[keep], g0.cols[keep]):
+            f.write(f"{a}\t{b}\n")
+    argv = ["-f", str(path), "--mcmc.input.contiguous", "-K", "8", "-n", "16", "-m", "300",
+            "-x", "61", "-i", "20", "--mcmc.seed", "3"]
+    r = subprocess.run([driver] + argv, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    pat = r"step count: (\d+) time: [0-9.]+ perplexity for hold out set: ([0-9.eE+-]+)"
+    got = [(int(m.group(1)), float(m.group(2))) for m in re.finditer(pat, r.stdout)]
+    opts = M.Options.parse(argv)
+    g = M.Graph.from_edge_list_file(str(path), contiguous=True)
+    smp = M.MCMCSamplerStochastic(opts, g)
+    smp.init()
+    ref = smp.run(print_log=False)
+    smp.close()
+    assert [t for t, _ in got] == [t for t, _ in ref] == [1, 21, 41]
+    for (_, a), (_, b) in zip(got, ref):
+        assert abs(a - b) <= 1e-11, (a, b)
+
+
 def test_trajectory_n5000_within_one_percent():
     """The reference's CPU-sized workload (N=5000, K=32, n=32, a=0.01,
     b=1024, c=0.55) from the same initial state and Philox seed: held-out

@@ -55,7 +55,66 @@ def test_options_parse_flags_aliases_config(tmp_path):
     assert a == pytest.approx(1024 ** -0.5) and alpha == 1 / 64
     with pytest.raises(ValueError):
         Options.parse(["--mcmc.strategy", "nope"])
-    assert parse_size("2M") == 2 * 1024 ** 2 and parse_size("3kB") == 3000
+    # the reference's parse_size_t: powers of 1024 for k/kb..., 0x hex, 0 octal
+    assert parse_size("2M") == 2 * 1024 ** 2 and parse_size("3kB") == 3 * 1024
+    assert parse_size("0x1b") == 27 and parse_size("010") == 8 and parse_size("1g") == 1 << 30
+    for bad in ("-5", "1q", "2kk"):
+        with pytest.raises(ValueError):
+            parse_size(bad)
+
+
+def _native_parse(argv):
+    """ammsb_options_parse (the C++ restatement) on argv."""
+    from mcmc_for_ammsb_amd import _lib
+    import ctypes
+    arr = (ctypes.c_char_p * max(1, len(argv)))(*[a.encode() for a in argv])
+    o, inp, n = _lib.Options(), _lib.Input(), ctypes.c_int32()
+    rc = _lib.lib.ammsb_options_parse(len(argv), arr, ctypes.byref(o), ctypes.byref(inp),
+                                      ctypes.byref(n))
+    return rc, o, inp, n.value
+
+
+def test_native_options_parser_matches_python(tmp_path):
+    """The C++ parser (for C/C++ hosts replacing the reference's main) and
+    Options.parse agree field by field, including the config file, aliases,
+    the reference's integer forms and the count of unrecognised arguments."""
+    cfg = tmp_path / "c.cfg"
+    cfg.write_text("# comment\n[mcmc]\nK = 16\nepsilon = 1e-6\ninterval=10\n"
+                   "input.compressed = true\nseed = 7\n")
+    cases = [
+        ["--mcmc.K", "32", "-m", "500", "--mcmc.num-node-sample=32", "--mcmc.a", "0.01",
+         "--mcmc.c=0.55", "--mcmc.strategy", "stratified-random-node-links",
+         "--mcmc.sampler.max-source", "8", "-x", "1k", "--unknown", "7", "pos"],
+        ["-K32", "-n0x10", "-i", "010", "--mcmc.seed=-3", "--mcmc.held-out-ratio", "0.2",
+         "-f", "/data/x.txt.gz", "-c", "rz", "--mcmc.input.contiguous", "--mcmc.precision",
+         "fp32", "--verbose", "--mcmc.stats", "2k", "--mcmc.strategy", "random-edge"],
+        ["--config", str(cfg), "-K", "64", "--mcmc.eta0=2.5"],
+        [],
+    ]
+    fields = ("alpha", "eta0", "eta1", "epsilon", "a", "b", "c", "K", "mini_batch_size",
+              "num_node_sample", "sampler_max_source", "max_iteration", "interval",
+              "stats_print_interval", "num_updates", "held_out_ratio", "random_seed",
+              "convergence_threshold", "device")
+    for argv in cases:
+        py = Options.parse(argv)
+        rc, o, inp, nun = _native_parse(argv)
+        assert rc == 0, argv
+        for f in fields:
+            assert getattr(o, f) == getattr(py, f), (argv, f)
+        assert o.strategy == M.options.STRATEGIES[py.strategy]
+        assert o.precision == (0 if py.precision == "fp64" else 1)
+        assert o.verbose == int(py.verbose)
+        assert inp.filename.decode() == py.input_filename
+        assert inp.input_class.decode() == py.input_class
+        assert (inp.contiguous, inp.compressed) == (int(py.input_contiguous),
+                                                    int(py.input_compressed))
+        assert nun == len(py.remains)
+    assert _native_parse(cases[2])[1].K == 64  # the command line (alias) wins over the file
+    for bad in (["--mcmc.K", "-5"], ["--mcmc.strategy", "nope"], ["-K", "1q"], ["--mcmc.K"],
+                ["--mcmc.precision", "fp16"]):
+        assert _native_parse(bad)[0] != 0, bad
+        with pytest.raises(ValueError):
+            Options.parse(bad)
 
 
 def _check_csr(g):
@@ -220,3 +279,21 @@ def test_reference_datasets_load(name):
     _check_csr(g)
     for a, b in list(pairs)[:200]:
         assert g.linked(a, b) and g.linked(b, a)
+
+
+DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "mcmc_for_ammsb_amd",
+                      "ammsb_mcmc")
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")
+def test_driver_option_and_input_errors(tmp_path):
+    """The command-line driver (the reference's main): --help, and exit code
+    33 for option and input errors (main.cc:79-86), all before any GPU use."""
+    import subprocess
+    r = subprocess.run([DRIVER, "--help"], capture_output=True, text=True)
+    assert r.returncode == 0 and "--mcmc.K" in r.stdout and r.stdout.startswith("Command line:")
+    for argv in (["--mcmc.K", "-5"], ["--mcmc.strategy", "nope"], ["-K", "8"],
+                 ["-f", str(tmp_path / "missing.txt")], ["-c", "netscience", "-f", "x.xml"]):
+        r = subprocess.run([DRIVER] + argv, capture_output=True, text=True)
+        assert r.returncode == 33, (argv, r.stdout, r.stderr)
+        assert "error" in r.stderr.lower()
