@@ -363,6 +363,52 @@ def batch_workload(n_chains, seed):
     return chains, graphs, splits, opts
 
 
+def run_operator(args):
+    """The injected-draw operator ammsb_update_phi (update_phi + update_pi of
+    given nodes with given neighbour lists and noise) called with host
+    buffers: the ids and the noise cross PCIe on every call, and the call
+    checks its ids on the host and allocates its device copies.  Reported
+    next to the resident kernel rate; never the headline value."""
+    import mcmc_for_ammsb_amd as M
+    w = WORKLOADS[args.workload]
+    g, s, opts = build_workload(w, args.seed)
+    smp = M.MCMCSamplerStochastic(opts, g, s)
+    smp.init(opts.random_seed)
+    inf = smp.info
+    N, K, nnb = inf["N"], inf["K"], inf["num_node_sample"] + 1
+    dt = np.float64 if opts.precision == "fp64" else np.float32
+    n_nodes, sets = 4096, 8
+    rng = np.random.default_rng(args.seed)
+    calls = []
+    for _ in range(sets):
+        nodes = rng.choice(N, n_nodes, replace=False).astype(np.int32)
+        nbrs = rng.integers(0, N, size=(n_nodes, nnb), dtype=np.int32)
+        noise = rng.standard_normal((n_nodes, K)).astype(dt)
+        calls.append((nodes, nbrs, noise))
+    out = {"metric": "injected update_phi node*K updates/s, host buffers (PCIe-inclusive)",
+           "unit": "node*K/s", "workload": w["desc"], "nodes_per_call": n_nodes,
+           "neighbours_per_node": nnb, "dtype": "f64" if dt is np.float64 else "f32"}
+    for label, with_noise in (("host_noise", True), ("device_noise", False)):
+        for i in range(args.warmup):
+            nodes, nbrs, noise = calls[i % sets]
+            smp.update_phi(nodes, nbrs, noise if with_noise else None, 0.01)
+        times = []
+        for i in range(args.steps):
+            nodes, nbrs, noise = calls[i % sets]
+            t = time.perf_counter()
+            smp.update_phi(nodes, nbrs, noise if with_noise else None, 0.01)
+            times.append(time.perf_counter() - t)
+        sec = float(np.median(times))
+        h2d = n_nodes * 4 + n_nodes * nnb * 4 + (n_nodes * K * dt().itemsize if with_noise else 0)
+        out[label] = {"ms_per_call": sec * 1e3, "value": n_nodes * K / sec,
+                      "host_to_device_bytes": h2d}
+        log(f"{label}: {sec * 1e3:.3f} ms per call, {n_nodes * K / sec:.3e} node*K/s, "
+            f"{h2d / 1e6:.1f} MB host to device")
+    smp.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def run_batch(args):
     """Batching workload: aggregate update_phi node*K/s of all chains, for
     the batched path (hipGraph replay, autotuned shapes) and for the same
@@ -453,7 +499,7 @@ def main():
     ap.add_argument("--phi-split", type=int, default=-1,
                     help="1: update_phi gather + finish kernels; 0: fused step + scatter")
     ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "update_phi_traffic.json"))
-    ap.add_argument("--mode", default="throughput", choices=["throughput", "ttt", "diff", "batch"])
+    ap.add_argument("--mode", default="throughput", choices=["throughput", "ttt", "diff", "batch", "operator"])
     ap.add_argument("--chains", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-separate", action="store_true")
@@ -476,6 +522,8 @@ def main():
         return run_diff(args)
     if args.mode == "batch":
         return run_batch(args)
+    if args.mode == "operator":
+        return run_operator(args)
 
     import mcmc_for_ammsb_amd as M
     w = WORKLOADS[args.workload]

@@ -14,7 +14,9 @@
 #include <chrono>
 #include <cmath>
 #include <deque>
+#include <initializer_list>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/ammsb.h"
@@ -130,6 +132,10 @@ struct ammsb_ctx {
   int32_t beta_blocks = 1;
   double* d_ppx_partials = nullptr;
   int32_t ppx_blocks = 1;
+  // device workspace of the injected-draw operators (ammsb_update_phi /
+  // _beta): grown on demand, kept with the context
+  char* d_opws = nullptr;
+  size_t opws_bytes = 0;
   // host mirrors of Learner counters
   int64_t step_count = 1;
   int64_t average_count = 1;
@@ -174,6 +180,31 @@ int dalloc(T** p, size_t count) {
     int _rc = dalloc(&(p), (count));  \
     if (_rc != AMMSB_OK) return _rc;  \
   } while (0)
+
+// Carves the operators' device buffers out of the context's workspace,
+// 256-byte aligned; grows it (after the stream has drained) when the call
+// needs more than it holds.
+int op_workspace(ammsb_ctx* c, std::initializer_list<std::pair<void**, size_t>> parts) {
+  size_t need = 0;
+  for (const auto& p : parts) need += (p.second + 255) / 256 * 256;
+  if (need > c->opws_bytes) {
+    if (c->d_opws) {
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      HIP_TRY(hipFree(c->d_opws));
+    }
+    c->d_opws = nullptr;
+    c->opws_bytes = 0;
+    const int rc = dalloc(&c->d_opws, need);
+    if (rc != AMMSB_OK) return rc;
+    c->opws_bytes = need;
+  }
+  char* q = c->d_opws;
+  for (const auto& p : parts) {
+    *p.first = p.second ? q : nullptr;
+    q += (p.second + 255) / 256 * 256;
+  }
+  return AMMSB_OK;
+}
 
 uint64_t pow2_at_least(uint64_t x) {
   uint64_t c = 1;
@@ -544,7 +575,7 @@ void free_ctx(ammsb_ctx* c) {
                   c->d_st,      c->ring.edges, c->ring.y,    c->ring.nodes,  c->ring.nbrs,
                   c->d_staging, c->d_mbtable, c->ring.stamp, c->d_nbr_gtable, c->d_beta_partials,
                   c->ring.meta, c->d_ll_scratch, c->d_bloom, c->d_esrc, c->d_re_keys, c->d_re_att,
-                  c->d_ppx_partials};
+                  c->d_ppx_partials, c->d_opws};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& p : c->pending) {
@@ -1184,29 +1215,12 @@ int ammsb_update_phi(ammsb_ctx* c, const int32_t* nodes, int64_t n_nodes, const 
   int32_t *d_nodes = nullptr, *d_nbrs = nullptr;
   void* d_noise = nullptr;
   void* d_stage = nullptr;
-  int rc = AMMSB_OK;
-  auto cleanup = [&]() {
-    if (d_nodes) (void)hipFree(d_nodes);
-    if (d_nbrs) (void)hipFree(d_nbrs);
-    if (d_noise) (void)hipFree(d_noise);
-    if (d_stage) (void)hipFree(d_stage);
-  };
-  do {
-    if ((rc = dalloc(&d_nodes, n_nodes)) != AMMSB_OK) break;
-    if ((rc = dalloc(&d_nbrs, n_nodes * n_nb)) != AMMSB_OK) break;
-    char* tmp = nullptr;
-    if ((rc = dalloc(&tmp, n_nodes * c->ld * c->fsize)) != AMMSB_OK) break;
-    d_stage = tmp;
-    if (noise) {
-      char* nz = nullptr;
-      if ((rc = dalloc(&nz, n_nodes * c->K * c->fsize)) != AMMSB_OK) break;
-      d_noise = nz;
-    }
-  } while (0);
-  if (rc != AMMSB_OK) {
-    cleanup();
-    return rc;
-  }
+  const int rc = op_workspace(
+      c, {{(void**)&d_nodes, (size_t)(4 * n_nodes)},
+          {(void**)&d_nbrs, (size_t)(4 * n_nodes * n_nb)},
+          {&d_stage, (size_t)(n_nodes * c->ld * c->fsize)},
+          {&d_noise, noise ? (size_t)(n_nodes * c->K * c->fsize) : (size_t)0}});
+  if (rc != AMMSB_OK) return rc;
   hipError_t e = hipMemcpy(d_nodes, nodes, 4 * n_nodes, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(d_nbrs, nbrs, 4 * n_nodes * n_nb, hipMemcpyHostToDevice);
   if (e == hipSuccess && noise)
@@ -1252,7 +1266,6 @@ int ammsb_update_phi(ammsb_ctx* c, const int32_t* nodes, int64_t n_nodes, const 
     e = launch_scatter(sc, c->prec, (int32_t)n_nodes, c->stream);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  cleanup();
   if (e != hipSuccess) {
     set_error(std::string("ammsb_update_phi: ") + hipGetErrorString(e));
     return AMMSB_ERR_HIP;
@@ -1274,9 +1287,9 @@ int ammsb_update_beta(ammsb_ctx* c, const int32_t* edges, const uint8_t* y, int6
   int32_t* d_edges = nullptr;
   uint8_t* d_y = nullptr;
   double* d_noise = nullptr;
-  int rc = dalloc(&d_edges, 2 * E);
-  if (rc == AMMSB_OK) rc = dalloc(&d_y, E);
-  if (rc == AMMSB_OK && noise) rc = dalloc(&d_noise, 2 * c->K);
+  const int rc = op_workspace(c, {{(void**)&d_edges, (size_t)(8 * E)},
+                                  {(void**)&d_y, (size_t)E},
+                                  {(void**)&d_noise, noise ? (size_t)(16 * c->K) : (size_t)0}});
   hipError_t e = hipSuccess;
   if (rc == AMMSB_OK) {
     if (E) e = hipMemcpy(d_edges, edges, 8 * E, hipMemcpyHostToDevice);
@@ -1320,9 +1333,6 @@ int ammsb_update_beta(ammsb_ctx* c, const int32_t* edges, const uint8_t* y, int6
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   }
-  if (d_edges) (void)hipFree(d_edges);
-  if (d_y) (void)hipFree(d_y);
-  if (d_noise) (void)hipFree(d_noise);
   if (rc != AMMSB_OK) return rc;
   if (e != hipSuccess) {
     set_error(std::string("ammsb_update_beta: ") + hipGetErrorString(e));

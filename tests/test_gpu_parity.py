@@ -322,6 +322,31 @@ def test_trajectory_n5000_within_one_percent():
     print("checkpoints", [(a, round(b, 6)) for a, b in gres[::5]], flush=True)
 
 
+def test_trajectory_n5000_fp32_within_one_percent():
+    """Precision mode on the same workload: the FP32 device run keeps the
+    held-out perplexity within 1% of the FP32 oracle (same operation order in
+    float, FP64 renormalisation and perplexity sums) and within 1% of the
+    FP64 oracle at every checkpoint of 1000 iterations."""
+    print("trajectory N=5000 K=32 fp32", flush=True)
+    N, K = 5000, 32
+    hyp = dict(n=32, m=500, a=0.01, b=1024.0, c=0.55, interval=50, seed=42)
+    g, s, smp, orc32 = make_pair("planted", N, K, precision="fp32", **hyp)
+    _, _, _, orc64 = make_pair("planted", N, K, precision="fp64", **hyp)
+    pi, theta = _state(N, K, 1)
+    smp.set_state(pi.astype(np.float32), theta)
+    orc32.set_state(pi.astype(np.float32), theta)
+    orc64.set_state(pi, theta)
+    gres = smp.run(max_steps=1000, print_log=False)
+    r32, _ = orc32.run(1000, 0, 500, 1, 50, 42)
+    r64, _ = orc64.run(1000, 0, 500, 1, 50, 42)
+    assert len(gres) == len(r32) == len(r64) == 20
+    for (gs, gp), (s32, p32), (s64, p64) in zip(gres, r32, r64):
+        assert gs == s32 == s64
+        assert abs(gp - p32) <= 0.01 * abs(p32), (gs, gp, p32)
+        assert abs(gp - p64) <= 0.01 * abs(p64), (gs, gp, p64)
+    print("checkpoints", [(a, round(b, 6)) for a, b in gres[::5]], flush=True)
+
+
 def test_init_state_properties():
     N, K = 3000, 64
     g, s = graph("community", N, K)
