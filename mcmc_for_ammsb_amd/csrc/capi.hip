@@ -124,6 +124,7 @@ struct ammsb_ctx {
   int32_t ll_pairs[2] = {0, 0}, ll_prefetch[2] = {0, 0};
   int32_t ll_fwd_rows = 1;  // forward of wide rows: whole rows per wavefront (else K-chunked)
   int32_t ll_bwd_rows = 1;  // adjoint of wide rows: the same (default FP64 only)
+  int32_t cus = 256;        // compute units of the device
   double* d_ll_scratch = nullptr;   // differentiation: per-source values / dbeta partials
   size_t ll_scratch_bytes = 0;
   int2* d_nbr_gtable = nullptr;
@@ -634,9 +635,16 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
   c->ll_pairs[1] = c->prec == AMMSB_FP64 ? 4 : 2;
   c->ll_prefetch[1] = c->prec == AMMSB_FP64 ? 0 : 1;
   // whole-row adjoint: faster for FP64 (N=1M, K=2048, 2.5M pairs: 8.6 -> 7.4
-  // ms), slower for FP32 (4.2 -> 4.8 ms: its LDS traffic per byte of row
+  // ms), slower for FP32 (4.16 -> 4.36 ms: its LDS traffic per byte of row
   // doubles)
   c->ll_bwd_rows = c->prec == AMMSB_FP64 ? 1 : 0;
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, o.device) ==
+            hipSuccess &&
+        cus > 0)
+      c->cus = cus;
+  }
   int lpr, ch;
   if (!row_config(c->K, c->prec, &lpr, &ch)) {
     set_error("ammsb_create: K too large for the device kernels (max 2048 FP64 / 4096 FP32)");
@@ -1517,11 +1525,19 @@ int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi
   HIP_TRY(hipSetDevice(c->opt.device));
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   if (stream) HIP_TRY(hipStreamSynchronize(c->stream));  // resident state settled
-  // the whole-row adjoint holds 96 KB of LDS per workgroup (one per CU) and
-  // writes a beta partial row per wavefront
+  // the whole-row adjoint holds (2 + 4) row-sized arrays in LDS per
+  // workgroup (FP64 K=2048: 96 KB, one workgroup per CU; FP32: 48 KB, three)
+  // and writes a beta partial row per wavefront: as many workgroups as are
+  // resident at once
   const bool rows_bwd = bwd && c->ll_bwd_rows && loglik_rows_chunks(c->K, c->prec) > 0;
-  const int32_t blocks =
-      (int32_t)std::max<int64_t>(1, std::min<int64_t>(M, rows_bwd ? 256 : 1024));
+  int64_t max_blocks = 1024;
+  if (rows_bwd) {
+    const int64_t per_cu = std::max<int64_t>(
+        1, std::min<int64_t>(4, (int64_t)kLdsPerCu /
+                                    (int64_t)(loglik_rows_bwd_lds(c->K, c->prec) + 256)));
+    max_blocks = per_cu * c->cus;
+  }
+  const int32_t blocks = (int32_t)std::max<int64_t>(1, std::min<int64_t>(M, max_blocks));
   const int64_t prows = rows_bwd ? (int64_t)blocks * kLlRowWaves : blocks;
   // scratch: per-source values (fwd) or [prows][K] dbeta partials (bwd), plus
   // one double for an absent total and the error word

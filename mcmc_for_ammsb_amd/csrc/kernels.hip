@@ -2385,6 +2385,12 @@ int loglik_kc_waves(int32_t K, int precision) {
   return w <= kLlMaxWaves ? w : 0;
 }
 
+size_t loglik_rows_bwd_lds(int32_t K, int precision) {
+  // the source row, the link factors and one adjoint array per wavefront:
+  // (2 + kLlRowWaves) arrays of NCH 16-byte vectors per lane
+  return (size_t)(2 + kLlRowWaves) * loglik_rows_chunks(K, precision) * 64 * 16;
+}
+
 int loglik_rows_chunks(int32_t K, int precision) {
   const int V = precision == 0 ? 2 : 4;
   const int vecs = (K + V - 1) / V;
@@ -2406,8 +2412,7 @@ hipError_t launch_loglik(const LogLikArgs& a, int precision, bool backward, hipS
   const int nch = loglik_rows_chunks(a.K, precision);
   if (rows && backward && nch > 0) {
     // a.blocks workgroups: a.blocks * kLlRowWaves beta partial rows
-    const int fs = precision == 0 ? 8 : 4;
-    const size_t lds = (size_t)(2 + kLlRowWaves) * nch * 64 * (16 / fs) * fs;
+    const size_t lds = loglik_rows_bwd_lds(a.K, precision);
     const dim3 grid(a.blocks), block(kLlRowWaves * 64);
 #define RB(NC)                                                                          \
   if (nch == NC) {                                                                      \
