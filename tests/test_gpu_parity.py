@@ -366,6 +366,48 @@ def test_init_state_properties():
     assert abs(pa[:, K].mean() - K) < 0.05 * K
 
 
+def test_empty_inputs_and_operator_reuse():
+    """Empty operator inputs: update_phi of no nodes changes nothing;
+    update_beta of no edges is the theta step with a zero gradient, which
+    has a closed form (mcmc_sampler_stochastic.cc:245-266 with G = 0).  Then
+    one context serves operator calls of growing and shrinking sizes (its
+    device workspace reused), each to the oracle at 1e-12."""
+    print("empty inputs / operator reuse", flush=True)
+    N, K = 4000, 32
+    g, s, smp, orc = make_pair("community", N, K, n=16)
+    pi, theta = _state(N, K, 5)
+    smp.set_state(pi, theta)
+    orc.set_state(pi, theta)
+    smp.update_phi(np.zeros(0, np.int32), np.zeros((0, 17), np.int32), None, 0.01)
+    gpi, gth, _ = smp.get_state()
+    assert np.array_equal(gpi, pi) and np.array_equal(gth, theta)
+    xi = np.random.default_rng(1).standard_normal((K, 2))
+    eps_t = 0.03
+    smp.update_beta(np.zeros((0, 2), np.int32), np.zeros(0, np.uint8), 1.0, xi, eps_t)
+    eta = np.array([1.0, 1.0])  # eta0, eta1 defaults
+    th = np.abs(theta + eps_t / 2.0 * (eta - theta) + np.sqrt(eps_t * theta) * xi)
+    th = np.maximum(th, 1e-24)
+    _, gth, gbe = smp.get_state()
+    ok, rel = _close(gth, th, TOL64)
+    assert ok, rel
+    ok, rel = _close(gbe, th[:, 1] / th.sum(1), TOL64)
+    assert ok, rel
+    orc.set_state(pi, gth)
+    rng = np.random.default_rng(2)
+    for n_nodes, nnb in ((300, 25), (20, 25), (40, 301), (300, 25)):
+        nodes = rng.choice(N, size=n_nodes, replace=False).astype(np.int32)
+        nbrs = np.stack([orc.sample_neighbors(int(v), nnb, 42, 7) & 0x7FFFFFFF for v in nodes])
+        noise = rng.standard_normal((n_nodes, K))
+        smp.update_phi(nodes, nbrs, noise, 0.011)
+        orc.update_phi(nodes, nbrs, noise, 0.011)
+        gpi, _, _ = smp.get_state()
+        opi, _, _ = orc.get_state()
+        ok, rel = _close(gpi[:, :K], opi[:, :K], TOL64)
+        assert ok, (n_nodes, nnb, rel)
+        ok, rel = _close(gpi[:, K], opi[:, K], TOL64)
+        assert ok, (n_nodes, nnb, rel)
+
+
 def test_error_paths():
     N, K = 2000, 16
     g, s = graph("community", N, K)
