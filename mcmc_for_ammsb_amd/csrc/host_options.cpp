@@ -164,10 +164,14 @@ struct Parser {
   bool set(const Flag& f, const std::string& val) {
     char* base = reinterpret_cast<char*>(o);
     switch (f.kind) {
-      case K_FLOAT: {
+      case K_FLOAT: {  // as lexical_cast<double>: no blanks, hexadecimal or nan(...) forms
+        bool plain = !val.empty();
+        for (const char ch : val)
+          if (isspace((unsigned char)ch) || ch == 'x' || ch == 'X' || ch == '(') plain = false;
+        if (!plain) break;
         char* end = nullptr;
         const double v = strtod(val.c_str(), &end);
-        if (!end || *end != '\0' || val.empty()) break;
+        if (!end || *end != '\0') break;
         memcpy(base + f.offset, &v, sizeof v);
         return true;
       }

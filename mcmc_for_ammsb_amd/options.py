@@ -67,6 +67,16 @@ def _parse_integral(text, lo, hi):
     return v
 
 
+def parse_real(text):
+    """A floating-point option, read as lexical_cast<double> reads it: the
+    whole text one decimal number (sign, digits, point, exponent) or
+    inf/nan; no blanks, no hexadecimal form, no digit separators."""
+    s = str(text)
+    if not s or any(ch.isspace() or ch in "_xX(" for ch in s):
+        raise ValueError(f"invalid number {s!r}")
+    return float(s)
+
+
 def parse_size(text):
     """A size_t option (no negative values)."""
     return _parse_integral(text, 0, (1 << 63) - 1)
@@ -112,9 +122,9 @@ class Options:
 
     # flag name -> (field, converter); short aliases of options.h
     _FLAGS = {
-        "mcmc.alpha": ("alpha", float), "mcmc.eta0": ("eta0", float),
-        "mcmc.eta1": ("eta1", float), "mcmc.epsilon": ("epsilon", float),
-        "mcmc.a": ("a", float), "mcmc.b": ("b", float), "mcmc.c": ("c", float),
+        "mcmc.alpha": ("alpha", parse_real), "mcmc.eta0": ("eta0", parse_real),
+        "mcmc.eta1": ("eta1", parse_real), "mcmc.epsilon": ("epsilon", parse_real),
+        "mcmc.a": ("a", parse_real), "mcmc.b": ("b", parse_real), "mcmc.c": ("c", parse_real),
         "mcmc.K": ("K", parse_size), "K": ("K", parse_size),
         "mcmc.mini-batch-size": ("mini_batch_size", parse_size), "m": ("mini_batch_size", parse_size),
         "mcmc.num-node-sample": ("num_node_sample", parse_size), "n": ("num_node_sample", parse_size),
@@ -124,9 +134,10 @@ class Options:
         "mcmc.interval": ("interval", parse_size), "i": ("interval", parse_size),
         "mcmc.stats": ("stats_print_interval", parse_size), "I": ("stats_print_interval", parse_size),
         "mcmc.num-updates": ("num_updates", parse_size),
-        "mcmc.held-out-ratio": ("held_out_ratio", float), "h": ("held_out_ratio", float),
+        "mcmc.held-out-ratio": ("held_out_ratio", parse_real),
+        "h": ("held_out_ratio", parse_real),
         "mcmc.seed": ("random_seed", parse_int),
-        "mcmc.convergence": ("convergence_threshold", float),
+        "mcmc.convergence": ("convergence_threshold", parse_real),
         "mcmc.input.file": ("input_filename", str), "f": ("input_filename", str),
         "mcmc.input.class": ("input_class", str), "c": ("input_class", str),
         "mcmc.precision": ("precision", str),

@@ -74,6 +74,50 @@ def _native_parse(argv):
     return rc, o, inp, n.value
 
 
+_INT_TEXT = "0123456789xXabcdefkKmMgGbB-"
+_FLOAT_TEXT = "0123456789.eE-+"
+
+
+def _py_parse(argv):
+    try:
+        return Options.parse(argv)
+    except ValueError:
+        return None
+
+
+@pytest.mark.parametrize("flag,field", [("--mcmc.max-iteration", "max_iteration"),
+                                        ("--mcmc.seed", "random_seed"),
+                                        ("--mcmc.epsilon", "epsilon")])
+def test_native_and_python_value_parsing_agree(flag, field):
+    """Random value strings (the reference's integer forms and their
+    mistakes, floats): the C++ and the Python parser accept the same ones
+    and agree on the value.  Integer options follow parse<T>
+    (options.h:39-57, options.cc:5-67), float options lexical_cast."""
+    hypothesis = pytest.importorskip("hypothesis")
+    from hypothesis import strategies as st
+
+    alphabet = _FLOAT_TEXT if field == "epsilon" else _INT_TEXT
+
+    @hypothesis.settings(max_examples=400, deadline=None, derandomize=True)
+    @hypothesis.given(st.text(alphabet=alphabet, min_size=1, max_size=12))
+    def check(text):
+        py = _py_parse([flag, text])
+        rc, o, _, _ = _native_parse([flag, text])
+        assert (rc == 0) == (py is not None), text
+        if py is not None:
+            assert getattr(o, field) == getattr(py, field), text
+
+    check()
+    for text in ("0x10", "010", "019", "1k", "2KB", "0x1gb", "-0", "-", "0x", "1m1",
+                 "18446744073709551615", "18446744073709551616", "-2147483648",
+                 "-2147483649", "2147483647", "2147483648", "17179869184g"):
+        py = _py_parse([flag, text])
+        rc, o, _, _ = _native_parse([flag, text])
+        assert (rc == 0) == (py is not None), text
+        if py is not None:
+            assert getattr(o, field) == getattr(py, field), text
+
+
 def test_native_options_parser_matches_python(tmp_path):
     """The C++ parser (for C/C++ hosts replacing the reference's main) and
     Options.parse agree field by field, including the config file, aliases,
