@@ -291,7 +291,10 @@ int ammsb_update_phi(ammsb_ctx *ctx, const int32_t *nodes, int64_t n_nodes,
                      const int32_t *nbrs, int64_t n_nb, const void *noise,
                      double eps_t);
 /* MCMCSamplerStochastic::update_beta (mcmc_sampler_stochastic.cc:200-267):
- * edges [E][2], y [E] in {0,1}, noise [K][2] double (NULL = Philox). */
+ * edges [E][2], y [E] in {0,1}, noise [K][2] double (NULL = Philox).
+ * Both operators take host arrays, check the ids on the host, copy them into
+ * a device workspace kept with the context (grown when a call needs more)
+ * and return after the device work has finished. */
 int ammsb_update_beta(ammsb_ctx *ctx, const int32_t *edges, const uint8_t *y,
                       int64_t E, double scale, const double *noise,
                       double eps_t);
