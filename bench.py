@@ -35,16 +35,16 @@ HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
     # name: (N, K, mean_degree, n, m, precision)
-    "c2": dict(N=1_000_000, K=256, deg=50.0, n=128, m=4095, precision="fp64",
+    "n1m-k256-fp64": dict(N=1_000_000, K=256, deg=50.0, n=128, m=4095, precision="fp64",
                desc="N=1M K=256 FP64 mean-degree-50 community graph, 4096-node minibatch, "
                     "128 neighbour samples"),
-    "c3": dict(N=4_000_000, K=1024, deg=40.0, n=128, m=4095, precision="fp32",
+    "n4m-k1024-fp32": dict(N=4_000_000, K=1024, deg=40.0, n=128, m=4095, precision="fp32",
                desc="N=4M K=1024 FP32 precision mode, mean-degree-40 community graph, "
                     "4096-node minibatch, 128 neighbour samples"),
-    "c3-fp64": dict(N=4_000_000, K=1024, deg=40.0, n=128, m=4095, precision="fp64",
+    "n4m-k1024-fp64": dict(N=4_000_000, K=1024, deg=40.0, n=128, m=4095, precision="fp64",
                     desc="N=4M K=1024 FP64, mean-degree-40 community graph, 4096-node "
                          "minibatch, 128 neighbour samples"),
-    "c1": dict(N=5000, K=32, deg=None, n=32, m=500, precision="fp64",
+    "n5000-k32-fp64": dict(N=5000, K=32, deg=None, n=32, m=500, precision="fp64",
                desc="N=5000 K=32 FP64 planted graph, m=500, 32 neighbour samples"),
 }
 
@@ -485,7 +485,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="n1m-k256-fp64", choices=sorted(WORKLOADS))
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU sample")
