@@ -67,7 +67,8 @@ def make_pair(kind, N, K, precision="fp64", n=32, m=None, strategy="stratified-r
 @pytest.mark.parametrize("split", [0, 1], ids=["fused", "split"])
 @pytest.mark.parametrize("kind,N,K", [("planted", 2000, 32), ("community", 20000, 256),
                                       ("community", 8000, 300), ("community", 4000, 7),
-                                      ("community", 3000, 1024), ("community", 3000, 1)])
+                                      ("community", 3000, 1024), ("community", 3000, 1),
+                                      ("community", 1500, 2048)])
 def test_update_phi_fp64_injected(kind, N, K, split):
     """split: gather kernel + k_phi_finish instead of the fused step."""
     print(f"update_phi fp64 {kind} N={N} K={K} split={split}", flush=True)
@@ -122,7 +123,7 @@ def test_update_phi_launch_shapes(waves, unroll, nnb):
     assert ok, rel
 
 
-@pytest.mark.parametrize("K", [32, 1024])
+@pytest.mark.parametrize("K", [32, 1024, 4096])
 def test_update_phi_fp32_injected(K):
     print(f"update_phi fp32 K={K}", flush=True)
     N = 4000
@@ -146,7 +147,7 @@ def test_update_phi_fp32_injected(K):
 @pytest.mark.parametrize("strategy,max_source", [("stratified-random-node-nonlinks", 1),
                                                  ("stratified-random-node-links", 1),
                                                  ("stratified-random-node-links", 8)])
-@pytest.mark.parametrize("K", [32, 256])
+@pytest.mark.parametrize("K", [32, 256, 2048])
 def test_update_beta_fp64_injected(strategy, max_source, K):
     print(f"update_beta fp64 {strategy} {max_source} K={K}", flush=True)
     N = 6000
@@ -168,7 +169,8 @@ def test_update_beta_fp64_injected(strategy, max_source, K):
     assert ok, rel
 
 
-@pytest.mark.parametrize("K,prec", [(32, "fp64"), (256, "fp64"), (300, "fp64"), (64, "fp32")])
+@pytest.mark.parametrize("K,prec", [(32, "fp64"), (256, "fp64"), (300, "fp64"), (2048, "fp64"),
+                                    (64, "fp32"), (4096, "fp32")])
 def test_perplexity_running_average(K, prec):
     print(f"perplexity K={K} {prec}", flush=True)
     N = 5000
@@ -421,6 +423,11 @@ def test_error_paths():
         smp.update_phi([N + 5], np.zeros((1, 4), np.int32), None, 0.01)
     with pytest.raises(M.AmmsbError):
         M.MCMCSamplerStochastic(M.Options(K=K, num_node_sample=N), g, s)
+    # the largest rows the device kernels take: K 2048 (FP64), 4096 (FP32)
+    for k_max, prec in ((2048, "fp64"), (4096, "fp32")):
+        M.MCMCSamplerStochastic(M.Options(K=k_max, num_node_sample=8, precision=prec), g, s).close()
+        with pytest.raises(M.AmmsbError):
+            M.MCMCSamplerStochastic(M.Options(K=k_max + 1, num_node_sample=8, precision=prec), g, s)
     smp.step(3)
     assert smp.info["step_count"] == 4
 
