@@ -340,7 +340,9 @@ int ammsb_get_totals(ammsb_ctx *ctx, int64_t *nodes_updated,
  * (8,0), adjoint (2,1).
  * ammsb_get_tuning also reads "sampler_lds" (read-only): bit 0 set when the
  * minibatch sampler's de-dup table lives in LDS, bit 1 when the neighbour
- * sampler's tables do (else they live in HBM). */
+ * sampler's tables do (else they live in HBM); and "ring_slots"
+ * (read-only): iterations of draws the sample-ahead ring holds (1 when one
+ * iteration's draws exceed the ring's 1 GiB budget). */
 int ammsb_set_tuning(ammsb_ctx *ctx, const char *name, int64_t value);
 int ammsb_get_tuning(ammsb_ctx *ctx, const char *name, int64_t *value);
 

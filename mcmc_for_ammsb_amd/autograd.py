@@ -31,18 +31,36 @@ def _check(t, name, dtype, shape=None):
         raise ValueError(f"{name} must be contiguous")
 
 
+def _check_args(sampler, phi, theta, nodes, pair_ptr, pair_nbr, pair_y, gout=None):
+    """Dtype, shape, contiguity and device of every argument, checked before
+    any native call (the kernels read and write these buffers by pointer).
+    Returns the number of sources M."""
+    K = sampler.info["K"]
+    fdt = torch.float64 if sampler.dtype == np.float64 else torch.float32
+    _check(nodes, "nodes", torch.int32)
+    if nodes.dim() != 1:
+        raise ValueError(f"nodes must be 1-d, got shape {tuple(nodes.shape)}")
+    M = nodes.shape[0]
+    _check(phi, "phi", fdt, (M, K))
+    _check(theta, "theta", torch.float64, (K, 2))
+    _check(pair_ptr, "pair_ptr", torch.int64, (M + 1,))
+    _check(pair_nbr, "pair_nbr", torch.int32)
+    if pair_nbr.dim() != 1:
+        raise ValueError(f"pair_nbr must be 1-d, got shape {tuple(pair_nbr.shape)}")
+    _check(pair_y, "pair_y", torch.uint8, pair_nbr.shape)
+    if gout is not None:
+        _check(gout, "gout", torch.float64, ())
+    devs = {t.device for t in (phi, theta, nodes, pair_ptr, pair_nbr, pair_y)
+            + ((gout,) if gout is not None else ())}
+    if len(devs) != 1:
+        raise ValueError(f"all tensors must be on one device, got {sorted(map(str, devs))}")
+    return M
+
+
 class LogLikelihood(torch.autograd.Function):
     @staticmethod
     def forward(ctx, phi, theta, sampler, nodes, pair_ptr, pair_nbr, pair_y):
-        K = sampler.info["K"]
-        fdt = torch.float64 if sampler.dtype == np.float64 else torch.float32
-        M = nodes.shape[0]
-        _check(phi, "phi", fdt, (M, K))
-        _check(theta, "theta", torch.float64, (K, 2))
-        _check(nodes, "nodes", torch.int32, (M,))
-        _check(pair_ptr, "pair_ptr", torch.int64, (M + 1,))
-        _check(pair_nbr, "pair_nbr", torch.int32)
-        _check(pair_y, "pair_y", torch.uint8, pair_nbr.shape)
+        M = _check_args(sampler, phi, theta, nodes, pair_ptr, pair_nbr, pair_y)
         per = torch.empty(M, dtype=torch.float64, device=phi.device)
         total = torch.empty((), dtype=torch.float64, device=phi.device)
         stream = torch.cuda.current_stream(phi.device).cuda_stream
@@ -59,6 +77,7 @@ class LogLikelihood(torch.autograd.Function):
     def backward(ctx, gout):
         phi, theta, nodes, pair_ptr, pair_nbr, pair_y = ctx.saved_tensors
         gout = gout.detach().to(torch.float64).contiguous().reshape(())
+        _check_args(ctx.sampler, phi, theta, nodes, pair_ptr, pair_nbr, pair_y, gout)
         grad_phi = torch.empty_like(phi)
         grad_theta = torch.empty_like(theta)
         stream = torch.cuda.current_stream(phi.device).cuda_stream
@@ -78,15 +97,13 @@ def loglik(sampler, phi, theta, nodes, pair_ptr, pair_nbr, pair_y):
 
 def loglik_per_source(sampler, phi, theta, nodes, pair_ptr, pair_nbr, pair_y):
     """The forward pass only, one FP64 value per source node (no autograd)."""
-    K = sampler.info["K"]
-    M = nodes.shape[0]
+    M = _check_args(sampler, phi, theta, nodes, pair_ptr, pair_nbr, pair_y)
     per = torch.empty(M, dtype=torch.float64, device=phi.device)
     stream = torch.cuda.current_stream(phi.device).cuda_stream
     _lib.check(_lib.lib.ammsb_loglik_fwd(
         sampler._h, M, nodes.data_ptr(), phi.data_ptr(), phi.stride(0), theta.data_ptr(),
         pair_ptr.data_ptr(), pair_nbr.data_ptr(), pair_y.data_ptr(), pair_nbr.shape[0],
         per.data_ptr(), None, stream))
-    assert phi.shape[1] == K
     return per
 
 
@@ -95,6 +112,7 @@ def loglik_grad(sampler, phi, theta, nodes, pair_ptr, pair_nbr, pair_y, gout=Non
     scaled by ``gout`` (a 0-d float64 GPU tensor, default 1)."""
     if gout is None:
         gout = torch.ones((), dtype=torch.float64, device=phi.device)
+    _check_args(sampler, phi, theta, nodes, pair_ptr, pair_nbr, pair_y, gout)
     grad_phi = torch.empty_like(phi)
     grad_theta = torch.empty_like(theta)
     stream = torch.cuda.current_stream(phi.device).cuda_stream

@@ -906,6 +906,43 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
   return AMMSB_OK;
 }
 
+// pi ++ phi_sum rows between the device layout [N][ld] and the host layout
+// [N][K+1]: contiguous pieces of whole device rows through a pinned staging
+// buffer, repacked on the host (a 2D copy of millions of short rows into
+// pageable memory is far slower).  Device padding past K+1 is written as 0.
+int transfer_state(ammsb_ctx* c, void* host, bool to_host) {
+  const size_t drow = (size_t)c->ld * c->fsize, hrow = (size_t)(c->K + 1) * c->fsize;
+  const int64_t rows_per = std::max<int64_t>(1, ((int64_t)64 << 20) / (int64_t)drow);
+  const int64_t nr_max = std::min<int64_t>(rows_per, c->N);
+  char* stage = nullptr;
+  HIP_TRY(hipHostMalloc((void**)&stage, (size_t)nr_max * drow, hipHostMallocDefault));
+  memset(stage, 0, (size_t)nr_max * drow);
+  hipError_t e = hipSuccess;
+  for (int64_t r0 = 0; r0 < c->N && e == hipSuccess; r0 += rows_per) {
+    const int64_t nr = std::min<int64_t>(rows_per, c->N - r0);
+    char* dev = c->d_pi + (size_t)r0 * drow;
+    char* h = (char*)host + (size_t)r0 * hrow;
+    if (to_host) {
+      e = hipMemcpyAsync(stage, dev, (size_t)nr * drow, hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e == hipSuccess)
+        parallel_for(nr, [&](int64_t lo, int64_t hi, int) {
+          for (int64_t r = lo; r < hi; ++r) memcpy(h + r * hrow, stage + r * drow, hrow);
+        }, 4096);
+    } else {
+      parallel_for(nr, [&](int64_t lo, int64_t hi, int) {
+        for (int64_t r = lo; r < hi; ++r) memcpy(stage + r * drow, h + r * hrow, hrow);
+      }, 4096);
+      e = hipMemcpyAsync(dev, stage, (size_t)nr * drow, hipMemcpyHostToDevice, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+  }
+  const hipError_t ef = hipHostFree(stage);
+  HIP_TRY(e);
+  HIP_TRY(ef);
+  return AMMSB_OK;
+}
+
 int set_beta_from_theta(ammsb_ctx* c, const double* theta) {
   std::vector<double> beta(c->K);
   for (int k = 0; k < c->K; ++k) {
@@ -1051,9 +1088,9 @@ int ammsb_set_state(ammsb_ctx* c, const void* pi, const double* theta) {
   if (!c || !pi || !theta) return AMMSB_ERR_ARG;
   HIP_TRY(hipSetDevice(c->opt.device));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy2D(c->d_pi, c->ld * c->fsize, pi, (c->K + 1) * c->fsize, (c->K + 1) * c->fsize,
-                      c->N, hipMemcpyHostToDevice));
-  int rc = set_beta_from_theta(c, theta);
+  int rc = transfer_state(c, const_cast<void*>(pi), false);
+  if (rc != AMMSB_OK) return rc;
+  rc = set_beta_from_theta(c, theta);
   if (rc != AMMSB_OK) return rc;
   c->state_ready = true;
   return AMMSB_OK;
@@ -1063,9 +1100,10 @@ int ammsb_get_state(ammsb_ctx* c, void* pi, double* theta, double* beta) {
   if (!c) return AMMSB_ERR_ARG;
   HIP_TRY(hipSetDevice(c->opt.device));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  if (pi)
-    HIP_TRY(hipMemcpy2D(pi, (c->K + 1) * c->fsize, c->d_pi, c->ld * c->fsize,
-                        (c->K + 1) * c->fsize, c->N, hipMemcpyDeviceToHost));
+  if (pi) {
+    const int rc = transfer_state(c, pi, true);
+    if (rc != AMMSB_OK) return rc;
+  }
   if (theta) HIP_TRY(hipMemcpy(theta, c->d_theta, sizeof(double) * 2 * c->K, hipMemcpyDeviceToHost));
   if (beta) HIP_TRY(hipMemcpy(beta, c->d_beta, sizeof(double) * c->K, hipMemcpyDeviceToHost));
   return AMMSB_OK;
@@ -1479,6 +1517,8 @@ int ammsb_get_tuning(ammsb_ctx* c, const char* name, int64_t* value) {
     *value = c->ll_bwd_rows;
   else if (n == "sampler_lds")  // read-only: bit 0 minibatch table, bit 1 neighbour tables in LDS
     *value = (c->mbtable_in_lds ? 1 : 0) | (c->nbr_use_lds ? 2 : 0);
+  else if (n == "ring_slots")  // read-only: iterations the sample-ahead ring holds
+    *value = c->ring.slots;
   else {
     set_error("ammsb_get_tuning: unknown parameter " + n);
     return AMMSB_ERR_ARG;

@@ -697,8 +697,14 @@ __device__ __forceinline__ void neighbors_body(const NeighborArgs& a, int32_t sl
   // LDS per wavefront: the de-dup table, then room for the node's row
   int2* tab = a.use_lds ? s_tab + (size_t)w * (a.tcap + kNbrRowLds / 2)
                         : a.gtable + (size_t)(slot - slot_offset) * a.tcap;
+  // A table in HBM is ordered by an agent-scope fence (its entries are
+  // global atomics, read back through L2); one in LDS by a workgroup fence.
+  const bool hbm_tab = !a.use_lds;
   for (int i = lane; i < a.tcap; i += 64) tab[i] = make_int2(-1, INT_MAX);
-  __threadfence_block();
+  if (hbm_tab)
+    __threadfence();
+  else
+    __threadfence_block();
   __builtin_amdgcn_wave_barrier();
   const uint32_t cmask = (uint32_t)(a.tcap - 1);
   const int64_t rs = a.g.row_ptr[node];
@@ -731,7 +737,10 @@ __device__ __forceinline__ void neighbors_body(const NeighborArgs& a, int32_t sl
       s = table_insert(tab, cmask, v);
       atomicMin(&tab[s].y, (int)ai);
     }
-    __threadfence_block();
+    if (hbm_tab)
+      __threadfence();
+    else
+      __threadfence_block();
     __builtin_amdgcn_wave_barrier();
     const bool win = valid && (a.use_lds ? tab[s].y : load_agent(&tab[s].y)) == (int)ai;
     const uint64_t mk = __ballot(win);

@@ -28,6 +28,15 @@
 //                    network.cc:237-268, 448-500, with the counter-based draw
 //                    addressing of this project (see below)
 //   orc_sample_neighbors   sample_neighbor_nodes, mcmc_sampler_stochastic.h:87-108
+//   orc_init_theta / orc_init_phi_row   init()'s state draw,
+//                    mcmc_sampler_stochastic.cc:54-76 (theta = gamma(eta0, eta1,
+//                    K, 2), GSL parameter order of Random::gamma,
+//                    random.cc:472-511; phi = gamma(1, 1, N, K); pi = phi
+//                    row-normalised)
+// With threads > 1, orc_run parallelises the loops the distributed reference
+// runs under OpenMP (DrawNeighbors, update_phi_node, beta_calc_grads,
+// cal_perplexity: mcmc_sampler_stochastic_distr.cc:1068,1142,1378,1543) with
+// per-thread accumulators summed in thread order.
 // Float is a template parameter (config.h.in:14-20); the code keeps the
 // reference's operation order and accumulation types (np::sum is a
 // sequential std::accumulate in Float, np.h:34-37; pi_from_phi accumulates
@@ -286,6 +295,21 @@ int sample_neighbors(const Net& g, const Stream& rs, int32_t node, int64_t nnb, 
   return 0;
 }
 
+// ---- threads ---------------------------------------------------------------
+// [0, n) in `threads` contiguous chunks (the static schedule of the
+// distributed reference's `#pragma omp parallel for` loops); fn(lo, hi, tid).
+template <typename Fn>
+void parallel_chunks(int64_t n, int threads, Fn fn) {
+  if (threads <= 1 || n < 2) {
+    fn((int64_t)0, n, 0);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int tt = 0; tt < threads; ++tt)
+    th.emplace_back(fn, n * tt / threads, n * (tt + 1) / threads, tt);
+  for (auto& x : th) x.join();
+}
+
 // ---- state ---------------------------------------------------------------
 struct Hyper {
   double alpha, eta0, eta1, epsilon, a, b, c;
@@ -374,7 +398,8 @@ void update_phi_set(State<Float>& S, const int32_t* nodes, int64_t n, const int3
                     int threads) {
   const int64_t K = S.K;
   std::vector<std::vector<Float>> out(n);
-  auto work = [&](int64_t lo, int64_t hi) {
+  // distr.cc:1142-1150 (omp parallel over the nodes)
+  parallel_chunks(n, threads, [&](int64_t lo, int64_t hi, int) {
     std::vector<Float> nz(K);
     for (int64_t q = lo; q < hi; ++q) {
       const Float* nzp;
@@ -386,51 +411,56 @@ void update_phi_set(State<Float>& S, const int32_t* nodes, int64_t n, const int3
       }
       update_phi_node<Float>(S, nodes[q], nbrs + q * nnb, nnb, nzp, eps_t, &out[q]);
     }
-  };
-  if (threads <= 1 || n < 2) {
-    work(0, n);
-  } else {
-    std::vector<std::thread> th;
-    for (int tt = 0; tt < threads; ++tt)
-      th.emplace_back(work, n * tt / threads, n * (tt + 1) / threads);
-    for (auto& x : th) x.join();
-  }
+  });
   for (int64_t q = 0; q < n; ++q) pi_from_phi<Float>(S.row(nodes[q]), out[q], K);
 }
 
 // update_beta (mcmc_sampler_stochastic.cc:200-267) with injected y and noise.
+// threads > 1: per-thread gradient rows over contiguous edge ranges, summed in
+// thread order (beta_calc_grads / beta_sum_grads, distr.cc:1377-1426).
 template <typename Float>
 void update_beta(State<Float>& S, const int32_t* edges, const uint8_t* ys, int64_t E,
-                 Float scale, const Float* noise /* [K][2] */, Float eps_t) {
+                 Float scale, const Float* noise /* [K][2] */, Float eps_t, int threads = 1) {
   const int64_t K = S.K;
   const Float epsilon = (Float)S.hp.epsilon;
-  std::vector<std::vector<Float>> grads(K, std::vector<Float>(2, (Float)0.0));
-  std::vector<Float> probs(K);
   std::vector<Float> theta_sum(K);
   for (int64_t k = 0; k < K; ++k)
     theta_sum[k] = std::accumulate(S.theta[k].begin(), S.theta[k].end(), static_cast<Float>(0));
-  for (int64_t e = 0; e < E; ++e) {
-    const int y = ys[e] ? 1 : 0;
-    const Float* pi_i = S.row(edges[2 * e]);
-    const Float* pi_j = S.row(edges[2 * e + 1]);
-    Float pi_sum = (Float)0.0;
-    for (int64_t k = 0; k < K; ++k) {
-      pi_sum += pi_i[k] * pi_j[k];
-      Float f = pi_i[k] * pi_j[k];
-      if (y == 1)
-        probs[k] = S.beta[k] * f;
-      else
-        probs[k] = ((Float)1.0 - S.beta[k]) * f;
+  const int nt = std::max(1, std::min<int>(threads, (int)std::max<int64_t>(E, 1)));
+  std::vector<std::vector<std::vector<Float>>> tg(
+      nt, std::vector<std::vector<Float>>(K, std::vector<Float>(2, (Float)0.0)));
+  parallel_chunks(E, nt, [&](int64_t lo, int64_t hi, int tid) {
+    std::vector<std::vector<Float>>& grads = tg[tid];
+    std::vector<Float> probs(K);
+    for (int64_t e = lo; e < hi; ++e) {
+      const int y = ys[e] ? 1 : 0;
+      const Float* pi_i = S.row(edges[2 * e]);
+      const Float* pi_j = S.row(edges[2 * e + 1]);
+      Float pi_sum = (Float)0.0;
+      for (int64_t k = 0; k < K; ++k) {
+        pi_sum += pi_i[k] * pi_j[k];
+        Float f = pi_i[k] * pi_j[k];
+        if (y == 1)
+          probs[k] = S.beta[k] * f;
+        else
+          probs[k] = ((Float)1.0 - S.beta[k]) * f;
+      }
+      Float prob_0 = ((y == 1) ? epsilon : ((Float)1.0 - epsilon)) * ((Float)1.0 - pi_sum);
+      Float prob_sum = std::accumulate(probs.begin(), probs.end(), static_cast<Float>(0)) + prob_0;
+      for (int64_t k = 0; k < K; ++k) {
+        Float f = probs[k] / prob_sum;
+        Float one_over_theta_sum = (Float)1.0 / theta_sum[k];
+        grads[k][0] += f * (((Float)1.0 - y) / S.theta[k][0] - one_over_theta_sum);
+        grads[k][1] += f * (y / S.theta[k][1] - one_over_theta_sum);
+      }
     }
-    Float prob_0 = ((y == 1) ? epsilon : ((Float)1.0 - epsilon)) * ((Float)1.0 - pi_sum);
-    Float prob_sum = std::accumulate(probs.begin(), probs.end(), static_cast<Float>(0)) + prob_0;
+  });
+  std::vector<std::vector<Float>>& grads = tg[0];
+  for (int tt = 1; tt < nt; ++tt)
     for (int64_t k = 0; k < K; ++k) {
-      Float f = probs[k] / prob_sum;
-      Float one_over_theta_sum = (Float)1.0 / theta_sum[k];
-      grads[k][0] += f * (((Float)1.0 - y) / S.theta[k][0] - one_over_theta_sum);
-      grads[k][1] += f * (y / S.theta[k][1] - one_over_theta_sum);
+      grads[k][0] += tg[tt][k][0];
+      grads[k][1] += tg[tt][k][1];
     }
-  }
   const Float eta[2] = {(Float)S.hp.eta0, (Float)S.hp.eta1};
   for (int64_t k = 0; k < K; ++k) {
     for (int i = 0; i < 2; ++i) {
@@ -468,24 +498,37 @@ Float edge_likelihood(const Float* pi_a, const Float* pi_b, bool y, const std::v
 }
 
 // cal_perplexity (learner.cc:138-183) over the held-out list in its order
+// threads > 1: per-thread link / non-link accumulators over contiguous
+// ranges, summed in thread order (distr.cc:1543-1600).
 template <typename Float>
-Float perplexity(State<Float>& S) {
+Float perplexity(State<Float>& S, int threads = 1) {
   const Net& g = *S.g;
+  const int64_t H = (int64_t)g.ho_labels.size();
+  const int nt = std::max(1, std::min<int>(threads, (int)std::max<int64_t>(H, 1)));
+  std::vector<Float> ll(nt, 0), nl(nt, 0);
+  std::vector<size_t> lc(nt, 0), nc(nt, 0);
+  parallel_chunks(H, nt, [&](int64_t lo, int64_t hi, int tid) {
+    for (int64_t i = lo; i < hi; ++i) {
+      const bool y = g.ho_labels[i] != 0;
+      Float L = edge_likelihood<Float>(S.row(g.ho_edges[2 * i]), S.row(g.ho_edges[2 * i + 1]), y,
+                                       S.beta, S.K, (Float)S.hp.epsilon);
+      S.ppx_R[i] = (S.ppx_R[i] * (S.average_count - 1) + L) / (S.average_count);
+      if (y) {
+        lc[tid]++;
+        ll[tid] += std::log(S.ppx_R[i]);
+      } else {
+        nc[tid]++;
+        nl[tid] += std::log(S.ppx_R[i]);
+      }
+    }
+  });
   Float link_likelihood = 0, non_link_likelihood = 0;
   size_t link_count = 0, non_link_count = 0;
-  const int64_t H = (int64_t)g.ho_labels.size();
-  for (int64_t i = 0; i < H; ++i) {
-    const bool y = g.ho_labels[i] != 0;
-    Float L = edge_likelihood<Float>(S.row(g.ho_edges[2 * i]), S.row(g.ho_edges[2 * i + 1]), y,
-                                     S.beta, S.K, (Float)S.hp.epsilon);
-    S.ppx_R[i] = (S.ppx_R[i] * (S.average_count - 1) + L) / (S.average_count);
-    if (y) {
-      link_count++;
-      link_likelihood += std::log(S.ppx_R[i]);
-    } else {
-      non_link_count++;
-      non_link_likelihood += std::log(S.ppx_R[i]);
-    }
+  for (int tt = 0; tt < nt; ++tt) {
+    link_likelihood += ll[tt];
+    non_link_likelihood += nl[tt];
+    link_count += lc[tt];
+    non_link_count += nc[tt];
   }
   Float avg = 0;
   if (link_count + non_link_count != 0)
@@ -564,7 +607,7 @@ int run_impl(Handle* h, int64_t iters, int strategy, int64_t m, int64_t max_sour
     const int64_t t = (int64_t)S.step_count;
     auto t0 = clk::now();
     if ((t - 1) % interval == 0) {
-      const double p = (double)perplexity<Float>(S);
+      const double p = (double)perplexity<Float>(S, threads);
       if (ppx_out && np < cap) {
         ppx_out[np] = p;
         ppx_steps[np] = t;
@@ -584,9 +627,16 @@ int run_impl(Handle* h, int64_t iters, int strategy, int64_t m, int64_t max_sour
     }
     auto t2 = clk::now();
     const Float eps_t = eps_t_of<Float>(S.hp, S.step_count);
+    // DrawNeighbors (distr.cc:1061-1100, omp parallel over the nodes); every
+    // node's draws are keyed by its id, so the split changes nothing
     std::vector<int32_t> nb((size_t)nodes.size() * nnb);
-    for (size_t q = 0; q < nodes.size(); ++q)
-      if (sample_neighbors(g, rs, nodes[q], nnb, t, &nb[q * nnb]) != 0) return -2;
+    std::vector<int> nb_rc(std::max(threads, 1), 0);
+    parallel_chunks((int64_t)nodes.size(), threads, [&](int64_t lo, int64_t hi, int tid) {
+      for (int64_t q = lo; q < hi; ++q)
+        if (sample_neighbors(g, rs, nodes[q], nnb, t, &nb[q * nnb]) != 0) nb_rc[tid] = -2;
+    });
+    for (int rc : nb_rc)
+      if (rc != 0) return rc;
     auto t3 = clk::now();
     update_phi_set<Float>(S, nodes.data(), (int64_t)nodes.size(), nb.data(), nnb, nullptr, eps_t,
                           &rs, t, threads);
@@ -599,7 +649,7 @@ int run_impl(Handle* h, int64_t iters, int strategy, int64_t m, int64_t max_sour
       tn[2 * k + 1] = z1;
     }
     update_beta<Float>(S, edges.data(), ys.data(), (int64_t)ys.size(), (Float)scale, tn.data(),
-                       eps_t);
+                       eps_t, threads);
     auto t5 = clk::now();
     S.step_count++;
     if (phase_secs) {
@@ -865,6 +915,35 @@ void orc_theta_noise(uint64_t seed, int64_t step, int64_t K, double* out) {
 
 double orc_log_gamma(uint64_t seed, double shape, uint32_t id, uint32_t iter, uint32_t tag) {
   return log_gamma(Stream(seed), shape, id, iter, tag);
+}
+
+// init() state draw (mcmc_sampler_stochastic.cc:54-76) on the counter
+// stream: theta[k][i] = Gamma(shape eta0, scale eta1) = exp(log G) * eta1
+// with G ~ Gamma(eta0, 1) from stream (2k+i, 0, INIT_THETA) -- the GSL order
+// of Random::gamma(a, b) is shape a, scale b (random.cc:472-511).
+void orc_init_theta(uint64_t seed, double eta0, double eta1, int64_t K, double* theta) {
+  const Stream rs(seed);
+  for (int64_t k = 0; k < K; ++k)
+    for (int i = 0; i < 2; ++i)
+      theta[2 * k + i] =
+          std::exp(log_gamma(rs, eta0, (uint32_t)(2 * k + i), 0, T_INIT_THETA)) * eta1;
+}
+
+// phi[node][k] ~ Gamma(1, 1) = -log U, U from block (k>>1, node, 0, INIT_PHI):
+// words (0,1) for even k, (2,3) for odd k; row = phi / sum(phi) ++ sum
+// (np::row_normalize, mcmc_sampler_stochastic.cc:66-76).  row: K+1 doubles.
+void orc_init_phi_row(uint64_t seed, int32_t node, int64_t K, double* row) {
+  const Stream rs(seed);
+  std::vector<double> phi(K);
+  for (int64_t k = 0; k < K; ++k) {
+    const Blk b = rs.blk((uint32_t)(k >> 1), (uint32_t)node, 0, T_INIT_PHI);
+    const uint64_t u = (k & 1) ? ((uint64_t)b.v[2] | ((uint64_t)b.v[3] << 32))
+                               : ((uint64_t)b.v[0] | ((uint64_t)b.v[1] << 32));
+    phi[k] = -std::log(unit53(u));
+  }
+  const double s = std::accumulate(phi.begin(), phi.end(), 0.0);
+  for (int64_t k = 0; k < K; ++k) row[k] = phi[k] / s;
+  row[K] = s;
 }
 
 // Full loop (the host replica of the device iteration); perplexity at

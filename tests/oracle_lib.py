@@ -39,6 +39,8 @@ def _load():
         "orc_phi_noise": (None, [c_i32, c_u64, c_i32, c_i64, c_i64, c_vp]),
         "orc_theta_noise": (None, [c_u64, c_i64, c_i64, P(c_dbl)]),
         "orc_log_gamma": (c_dbl, [c_u64, c_dbl, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+        "orc_init_theta": (None, [c_u64, c_dbl, c_dbl, c_i64, P(c_dbl)]),
+        "orc_init_phi_row": (None, [c_u64, c_i32, c_i64, P(c_dbl)]),
         "orc_phi_grads": (None, [c_vp, c_i32, P(c_i32), c_i64, P(c_dbl)]),
         "orc_beta_grads": (None, [c_vp, P(c_i32), P(ctypes.c_uint8), c_i64, P(c_dbl)]),
         "orc_run": (c_i32, [c_vp, c_i64, c_i32, c_i64, c_i64, c_i64, c_u64, c_i32, P(c_dbl), P(c_i64),
@@ -64,6 +66,23 @@ def philox(ctr, key):
     o = np.zeros(4, dtype=np.uint32)
     lib.orc_philox(_p(c, ctypes.c_uint32), _p(k, ctypes.c_uint32), _p(o, ctypes.c_uint32))
     return o
+
+
+def init_theta(seed, eta0, eta1, K):
+    """theta [K][2] of init() (mcmc_sampler_stochastic.cc:54): Gamma(shape
+    eta0, scale eta1) per entry on the counter stream."""
+    out = np.zeros((K, 2))
+    lib.orc_init_theta(seed, eta0, eta1, K, _p(out, c_dbl))
+    return out
+
+
+def init_phi_rows(seed, nodes, K):
+    """pi ++ phi_sum rows [len(nodes)][K+1] of init() (phi ~ Gamma(1,1),
+    mcmc_sampler_stochastic.cc:66-76), in FP64."""
+    out = np.zeros((len(nodes), K + 1))
+    for i, v in enumerate(nodes):
+        lib.orc_init_phi_row(seed, int(v), K, _p(out[i], c_dbl))
+    return out
 
 
 def hyper_array(alpha, eta0, eta1, epsilon, a, b, c, num_node_sample):

@@ -163,6 +163,41 @@ def test_bad_neighbour_id_is_an_error(setup):
         M.autograd.loglik(smp, phi_t, th_t, n_t, p_t, b_t, y_t)
 
 
+def test_argument_checks_before_native_calls(setup):
+    """Every entry of the torch front end (the autograd function and the
+    forward-only / adjoint-only helpers) checks dtype, shape, contiguity and
+    device before any native call: a phi of the other precision or of the
+    wrong width, or a non-contiguous or 2-d input, is an error and nothing
+    is launched."""
+    g, s, pi, theta, nodes, ptr, nbr, y = setup
+    smp = _sampler(g, s, pi, theta)
+    phi_t, th_t, n_t, p_t, b_t, y_t = _inputs(pi, theta, nodes, ptr, nbr, y)
+    phi_t, th_t = phi_t.detach(), th_t.detach()
+    K = theta.shape[0]
+    bad_args = [
+        ((phi_t.float(), th_t, n_t, p_t, b_t, y_t), TypeError),           # FP32 phi, FP64 sampler
+        ((phi_t[:, :K - 1].contiguous(), th_t, n_t, p_t, b_t, y_t), ValueError),  # narrow rows
+        ((phi_t[:-1], th_t, n_t, p_t, b_t, y_t), ValueError),            # rows != sources
+        ((phi_t, th_t.float(), n_t, p_t, b_t, y_t), TypeError),
+        ((phi_t, th_t, n_t, p_t[:-1], b_t, y_t), ValueError),
+        ((phi_t, th_t, n_t, p_t, b_t, y_t[:-1]), ValueError),
+        ((phi_t, th_t, n_t, p_t, b_t.reshape(1, -1), y_t.reshape(1, -1)), ValueError),
+        ((phi_t.t().contiguous().t(), th_t, n_t, p_t, b_t, y_t), ValueError),  # not contiguous
+        ((phi_t.cpu(), th_t, n_t, p_t, b_t, y_t), TypeError),
+    ]
+    for args, exc in bad_args:
+        for fn in (M.autograd.loglik_per_source, M.autograd.loglik_grad, M.autograd.loglik):
+            with pytest.raises(exc):
+                fn(smp, *args)
+    with pytest.raises(TypeError):
+        M.autograd.loglik_grad(smp, phi_t, th_t, n_t, p_t, b_t, y_t,
+                               gout=torch.ones((), dtype=torch.float32, device=phi_t.device))
+    # valid arguments still work after the rejected calls
+    per = M.autograd.loglik_per_source(smp, phi_t, th_t, n_t, p_t, b_t, y_t)
+    assert torch.isfinite(per).all()
+    smp.close()
+
+
 def test_kchunk_launch_shapes_agree(setup):
     """Every (ll_pairs, ll_prefetch) shape of the K-chunked kernels sums in
     the same order, so values and gradients are bit-identical; the forward

@@ -6,6 +6,8 @@ Pins:
     run from the same state and Philox seed: 1e-12 relative (draws are the
     chain's own Philox streams, so they equal the oracle's sequential
     replica bit for bit)
+  * every chain of the FP32 batch (K up to 512) after one and three
+    iterations against the FP32 oracle: 1e-4 relative
   * FP32 batch against single contexts with the same options and state:
     1e-4 relative after three iterations (summation order differs)
   * hipGraph replay against direct launches: bit-identical
@@ -87,6 +89,30 @@ def test_batch_fp64_one_iteration_matches_oracle():
         assert _close(gth, rth, 1e-12), (i, "theta")
         assert _close(gbeta, rbeta, 1e-12), (i, "beta")
         assert b.chain(i).info["step_count"] == 2
+    b.close()
+
+
+@pytest.mark.parametrize("iters", [1, 3])
+def test_batch_fp32_matches_oracle(iters):
+    """The batching configuration's own precision: every chain of the uneven
+    FP32 batch (K 16 ... 512) after one and after three iterations equals
+    the FP32 oracle's run() (the reference loop per chain,
+    mcmc_sampler_stochastic.cc:93-183, in float with FP64 renormalisation)
+    from the same state and Philox seed, to 1e-4."""
+    print(f"batch fp32 vs oracle, {len(CHAINS)} chains, {iters} iterations", flush=True)
+    b, opts, graphs, splits, states = _batch("fp32", CHAINS)
+    b.step(iters)
+    assert max(c[1] for c in CHAINS) == 512
+    for i, (N, K, deg, n, m, seed) in enumerate(CHAINS):
+        orc = Oracle(graphs[i], splits[i], K, precision="fp32", num_node_sample=n)
+        orc.set_state(*states[i])
+        orc.run(iters, 0, m, 1, 10 ** 9, seed)
+        rpi, rth, rbeta = orc.get_state()
+        gpi, gth, gbeta = b.chain(i).get_state()
+        assert _close(gpi, rpi, 1e-4), (i, K, "pi")
+        assert _close(gth, rth, 1e-4), (i, K, "theta")
+        assert _close(gbeta, rbeta, 1e-4), (i, K, "beta")
+        assert b.chain(i).info["step_count"] == 1 + iters
     b.close()
 
 

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import mcmc_for_ammsb_amd as M
+from tests import oracle_lib
 from tests.oracle_lib import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -347,6 +348,18 @@ def test_trajectory_n5000_fp32_within_one_percent():
         assert abs(gp - p32) <= 0.01 * abs(p32), (gs, gp, p32)
         assert abs(gp - p64) <= 0.01 * abs(p64), (gs, gp, p64)
     print("checkpoints", [(a, round(b, 6)) for a, b in gres[::5]], flush=True)
+    # error of the precision mode against the FP64 oracle after 1000
+    # iterations, per quantity (pi, beta, perplexity over all checkpoints)
+    gpi, _, gbe = smp.get_state()
+    opi, _, obe = orc64.get_state()
+    dpi = np.abs(gpi[:, :K].astype(np.float64) - opi[:, :K])
+    rb = np.abs(gbe / obe - 1)
+    rp = np.array([abs(gp - p64) / p64 for (_, gp), (_, p64) in zip(gres, r64)])
+    print(f"fp32 vs fp64 oracle after 1000 iterations: pi max abs {dpi.max():.3e} mean abs "
+          f"{dpi.mean():.3e}; beta max rel {rb.max():.3e} mean rel {rb.mean():.3e}; "
+          f"perplexity max rel {rp.max():.3e} mean rel {rp.mean():.3e}", flush=True)
+    assert rp.max() <= 0.01
+    assert dpi.mean() <= 1e-2 and rb.mean() <= 0.1
 
 
 def test_init_state_properties():
@@ -366,6 +379,95 @@ def test_init_state_properties():
     assert np.allclose(ba, ta[:, 1] / ta.sum(1), rtol=1e-15)
     # Gamma(1,1) rows: E[phi_sum] = K
     assert abs(pa[:, K].mean() - K) < 0.05 * K
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("eta0,eta1", [(1.0, 1.0), (2.0, 0.5), (0.5, 3.0)])
+def test_init_state_matches_oracle(eta0, eta1, prec):
+    """init() (mcmc_sampler_stochastic.cc:54-76): theta = gamma(eta0, eta1,
+    K, 2) bit-exact against the oracle's exp(log Gamma(eta0)) * eta1 on the
+    same counters (shape eta0, scale eta1: random.cc:472-511; a swap changes
+    every value once eta0 != eta1), beta = theta1 / (theta0 + theta1), and
+    the pi ++ phi_sum rows against the oracle's -log U draws row-normalised
+    (FP64 1e-13: the device's log and summation order differ; FP32 rows are
+    the FP64 values rounded, 1e-6)."""
+    print(f"init eta=({eta0},{eta1}) {prec}", flush=True)
+    N, K, seed = 3000, 64, 77
+    g, s = graph("community", N, K)
+    smp = M.MCMCSamplerStochastic(M.Options(K=K, num_node_sample=10, eta0=eta0, eta1=eta1,
+                                            precision=prec), g, s)
+    smp.init(seed)
+    _, theta, beta = smp.get_state()
+    ref = oracle_lib.init_theta(seed, eta0, eta1, K)
+    np.testing.assert_array_equal(theta, ref)
+    np.testing.assert_array_equal(beta, ref[:, 1] / (ref[:, 0] + ref[:, 1]))
+    nodes = np.unique(np.concatenate([[0, N - 1], np.random.default_rng(1).choice(N, 200)]))
+    rows = smp.get_rows(nodes).astype(np.float64)
+    want = oracle_lib.init_phi_rows(seed, nodes, K)
+    ok, rel = _close(rows, want, 1e-13 if prec == "fp64" else 1e-6)
+    assert ok, rel
+    smp.close()
+
+
+@pytest.mark.parametrize("strategy,max_source", [("stratified-random-node-nonlinks", 1),
+                                                 ("stratified-random-node-links", 8)])
+@pytest.mark.parametrize("K", [32, 1024])
+def test_update_beta_fp32_injected(strategy, max_source, K):
+    """Precision mode: update_beta with FP32 rows and FP64 gradient
+    accumulation against the FP32 oracle (the reference's FP32 build,
+    accumulating in float) and the FP64 oracle on the same float rows, to
+    1e-4."""
+    print(f"update_beta fp32 {strategy} K={K}", flush=True)
+    N = 6000
+    strat = {"stratified-random-node-nonlinks": 2, "stratified-random-node-links": 1}[strategy]
+    g, s, smp, orc32 = make_pair("community", N, K, precision="fp32", n=16, m=600,
+                                 max_source=max_source)
+    orc64 = Oracle(g, s, K, precision="fp64", num_node_sample=16)
+    pi, theta = _state(N, K, 9, np.float32)
+    smp.set_state(pi, theta)
+    mb = orc64.sample_minibatch(strat, 600, max_source, 42, 3)
+    assert len(mb["y"]) > 0
+    noise = np.random.default_rng(3).standard_normal((K, 2))
+    smp.update_beta(mb["edges"], mb["y"], mb["scale"], noise, 0.02)
+    _, gth, gbe = smp.get_state()
+    for orc, state_pi in ((orc32, pi), (orc64, pi.astype(np.float64))):
+        orc.set_state(state_pi, theta)
+        orc.update_beta(mb["edges"], mb["y"], mb["scale"], noise, 0.02)
+        _, oth, obe = orc.get_state()
+        ok, rel = _close(gth, oth, 1e-4)
+        assert ok, (orc.prec, rel)
+        ok, rel = _close(gbe, obe, 1e-4)
+        assert ok, (orc.prec, rel)
+
+
+@pytest.mark.parametrize("flag", [0, 1], ids=["nonlinks", "links"])
+@pytest.mark.parametrize("max_source", [1, 16])
+def test_stratified_iteration_fp64(flag, max_source):
+    """One full FP64 iteration of the default strategy (stratified random
+    node) through ammsb_step -- samplers, update_phi + update_pi,
+    update_beta + theta step -- against the oracle's run() from the same
+    state, step and seed: 1e-12."""
+    N, K, seed = 6000, 32, 13
+    t = 2
+    while (int(oracle_lib.philox([0, 0, t, 1], [seed, 0])[0]) & 1) != flag:
+        t += 1
+    print(f"stratified iteration flag={flag} max_source={max_source} step={t}", flush=True)
+    g, s, smp, orc = make_pair("community", N, K, n=20, m=600, max_source=max_source, seed=seed,
+                               a=0.01, c=0.55)
+    pi, theta = _state(N, K, 6)
+    smp.set_state(pi, theta)
+    orc.set_state(pi, theta)
+    smp.set_counters(t, 1)
+    orc.set_counters(t, 1)
+    smp.step(1)
+    orc.run(1, 0, 600, max_source, 10 ** 9, seed)
+    gpi, gth, gbe = smp.get_state()
+    rpi, rth, rbe = orc.get_state()
+    assert np.sum(np.any(gpi != pi, axis=1)) > 1
+    for gv, rv in ((gpi, rpi), (gth, rth), (gbe, rbe)):
+        ok, err = _close(gv, rv, TOL64)
+        assert ok, err
+    assert smp.info["step_count"] == t + 1
 
 
 def test_empty_inputs_and_operator_reuse():

@@ -473,6 +473,67 @@ def test_gamma_and_normal_moments():
     assert abs(np.corrcoef(z[:, 0], z[:, 1])[0, 1]) < 0.03
 
 
+@pytest.mark.parametrize("eta0,eta1", [(1.0, 1.0), (2.0, 0.5), (0.5, 3.0)])
+def test_init_theta_is_gamma_shape_scale(eta0, eta1):
+    """init()'s theta = gamma(eta0, eta1, K, 2) (mcmc_sampler_stochastic.cc:54)
+    is Gamma(shape eta0, scale eta1) in the GSL order of Random::gamma
+    (random.cc:472-511): a Kolmogorov-Smirnov test against that law passes
+    and one against the swapped law (shape eta1, scale eta0) fails, so a swap
+    cannot go unnoticed.  Also the moments E = eta0*eta1, Var = eta0*eta1^2."""
+    from scipy import stats
+    th = oracle_lib.init_theta(7, eta0, eta1, 10000).ravel()
+    assert stats.kstest(th, stats.gamma(a=eta0, scale=eta1).cdf).pvalue > 1e-3
+    if eta0 != eta1:
+        assert stats.kstest(th, stats.gamma(a=eta1, scale=eta0).cdf).pvalue < 1e-6
+    n = len(th)
+    assert abs(th.mean() - eta0 * eta1) < 5 * np.sqrt(eta0 * eta1 ** 2 / n)
+    assert abs(th.var() / (eta0 * eta1 ** 2) - 1) < 0.1
+    # entry (k, i) is draw id 2k+i of the INIT_THETA stream
+    assert th[5] == np.exp(oracle_lib.lib.orc_log_gamma(7, eta0, 5, 0, 8)) * eta1
+
+
+def test_init_phi_rows_are_normalised_exponentials():
+    """init()'s phi = gamma(1, 1, N, K), pi = phi row-normalised
+    (mcmc_sampler_stochastic.cc:66-76): phi = pi * phi_sum is Exp(1)
+    (Kolmogorov-Smirnov), rows sum to 1, and the rows of different nodes are
+    different draws."""
+    from scipy import stats
+    K = 64
+    rows = oracle_lib.init_phi_rows(9, np.arange(300), K)
+    phi = rows[:, :K] * rows[:, K:]
+    assert stats.kstest(phi.ravel(), stats.expon().cdf).pvalue > 1e-3
+    assert np.allclose(rows[:, :K].sum(1), 1.0, atol=1e-14)
+    assert abs(rows[:, K].mean() - K) < 5 * np.sqrt(K / 300)
+    assert not np.array_equal(rows[0], rows[1])
+    # element k of node v is -log U from Philox block (k>>1, v, 0, INIT_PHI),
+    # words (0, 1) for even k and (2, 3) for odd k, U = (top 53 bits + 1/2) 2^-53
+    b = oracle_lib.philox([3 >> 1, 17, 0, 7], [9, 0])
+    u = (((int(b[2]) | (int(b[3]) << 32)) >> 11) + 0.5) / 2.0 ** 53
+    assert phi[17, 3] == pytest.approx(-np.log(u), rel=1e-14)
+
+
+def test_oracle_threads_match_sequential():
+    """orc_run with several threads (the distributed reference's OpenMP
+    loops: neighbour draws, update_phi, beta gradients, perplexity, with
+    per-thread accumulators) equals the one-thread loop up to summation
+    order."""
+    g = M.generate("community", N=3000, K=16, mean_degree=15.0, seed=5)
+    s = M.make_split(g, 0.1, 42)
+    res, states = [], []
+    for threads in (1, 4):
+        o = Oracle(g, s, 16, num_node_sample=30, a=0.01, b=1024.0, c=0.55)
+        pi, theta = state(g.N, 16, 3)
+        o.set_state(pi, theta)
+        r, _ = o.run(25, 0, 300, 1, 10, 42, threads=threads)
+        res.append(r)
+        states.append(o.get_state())
+    assert [t for t, _ in res[0]] == [t for t, _ in res[1]] == [1, 11, 21]
+    for (_, a), (_, b) in zip(res[0], res[1]):
+        assert abs(a - b) <= 1e-12 * abs(a)
+    for x, y in zip(states[0], states[1]):
+        np.testing.assert_allclose(x, y, rtol=1e-11, atol=1e-14)
+
+
 def test_oracle_loop_decreases_perplexity():
     g = M.generate("planted", N=2000, K=8, seed=11)
     s = M.make_split(g, 0.1, 42)
