@@ -376,6 +376,21 @@ int ammsb_loglik_bwd(ammsb_ctx *ctx, int64_t M, const int32_t *nodes, const void
                      int64_t phi_ld, const double *theta, const int64_t *pair_ptr,
                      const int32_t *pair_nbr, const uint8_t *pair_y, int64_t P,
                      const double *gout, void *grad_phi, double *grad_theta, void *stream);
+/* ammsb_loglik_bwd with no device allocation.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * adjoint's scratch -- its beta partial rows, 8 * rows * K bytes, and an
+ * error word -- lives in a file-scope device array of 32 MiB + 4 KiB per GPU,
+ * held until the process ends; a call that needs more is AMMSB_ERR_ARG
+ * (rows = the adjoint's workgroup partials: up to 1024, or 4 per workgroup
+ * of the whole-row FP64 kernel).  Same arguments, results and errors as
+ * ammsb_loglik_bwd otherwise.  All calls on one GPU share the array: calls
+ * that overlap on one GPU race. */
+int ammsb_loglik_bwd_noalloc(ammsb_ctx *ctx, int64_t M, const int32_t *nodes, const void *phi,
+                             int64_t phi_ld, const double *theta, const int64_t *pair_ptr,
+                             const int32_t *pair_nbr, const uint8_t *pair_y, int64_t P,
+                             const double *gout, void *grad_phi, double *grad_theta,
+                             void *stream);
 
 /* Device pointers of the resident state for interop (torch DLPack-free
  * views): pi ([N][ld] of the precision type), theta ([K][2] double),

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <deque>
 #include <initializer_list>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -22,6 +23,16 @@
 #include "../../include/ammsb.h"
 #include "host_util.h"
 #include "kernels.h"
+
+// Device scratch of the no-allocation adjoint (ammsb_loglik_bwd_noalloc):
+// a file-scope array, one copy per GPU, held until the process ends.  Its
+// first 256 bytes hold the error word, the rest the adjoint's beta partial
+// rows ([rows][K] doubles).  Host code never passes `scratch` itself to a
+// kernel (on the host it names a stand-in, not device memory): it uses the
+// per-device address from hipGetSymbolAddress (ammsb::noalloc_scratch).
+static constexpr size_t kNoAllocHeader = 256;
+static constexpr size_t kNoAllocScratchBytes = ((size_t)32 << 20) + 4096;  // 32 MiB + 4 KiB
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
 
 namespace ammsb {
 
@@ -173,6 +184,31 @@ int dalloc(T** p, size_t count) {
     *p = nullptr;
     return AMMSB_ERR_OOM;
   }
+  return AMMSB_OK;
+}
+
+// Device address of `scratch` on `device` (the current device), looked up
+// once per device and kept: every GPU holds its own copy of the array.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("no-allocation scratch: device ordinal out of range");
+    return AMMSB_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return AMMSB_ERR_HIP;
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
   return AMMSB_OK;
 }
 
@@ -689,6 +725,12 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
 
   HIP_TRY(hipSetDevice(o.device));
   if (!c->stream) HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  {
+    // the no-allocation adjoint's scratch address on this device, once
+    void* sp = nullptr;
+    const int rc = noalloc_scratch(o.device, &sp);
+    if (rc != AMMSB_OK) return rc;
+  }
 
   // ---- graph
   for (int32_t i = 0; i < c->N; ++i)
@@ -1553,7 +1595,8 @@ namespace {
 int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi, int64_t phi_ld,
                   const double* theta, const int64_t* pair_ptr, const int32_t* pair_nbr,
                   const uint8_t* pair_y, int64_t P, bool bwd, double* per_source, double* total,
-                  const double* gout, void* grad_phi, double* grad_theta, void* stream) {
+                  const double* gout, void* grad_phi, double* grad_theta, void* stream,
+                  bool noalloc = false) {
   if (!c || M < 0 || P < 0 || (M > 0 && (!nodes || !phi || !pair_ptr)) || (P > 0 && (!pair_nbr || !pair_y)) ||
       phi_ld < c->K) {
     set_error("ammsb_loglik: bad arguments");
@@ -1581,16 +1624,36 @@ int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi
   const int64_t prows = rows_bwd ? (int64_t)blocks * kLlRowWaves : blocks;
   // scratch: per-source values (fwd) or [prows][K] dbeta partials (bwd), plus
   // one double for an absent total and the error word
-  const int64_t body = std::max<int64_t>(M, prows * c->K);
-  const size_t need = sizeof(double) * (size_t)(body + 2);
-  if (need > c->ll_scratch_bytes) {
-    if (c->d_ll_scratch) HIP_TRY(hipFree(c->d_ll_scratch));
-    c->d_ll_scratch = nullptr;
-    c->ll_scratch_bytes = 0;
-    HIP_TRY(hipMalloc((void**)&c->d_ll_scratch, need));
-    c->ll_scratch_bytes = need;
+  const int64_t body = bwd ? prows * c->K : M;
+  double* scr = nullptr;
+  int32_t* d_err = nullptr;
+  if (noalloc) {
+    // the file-scope array: error word in the header, then the body
+    const size_t need = kNoAllocHeader + sizeof(double) * (size_t)body;
+    if (need > kNoAllocScratchBytes) {
+      set_error("ammsb_loglik_bwd_noalloc: the adjoint needs " + std::to_string(need) +
+                " bytes of scratch (" + std::to_string(prows) + " partial rows of K=" +
+                std::to_string(c->K) + "), more than the " +
+                std::to_string(kNoAllocScratchBytes) + " the no-allocation array holds");
+      return AMMSB_ERR_ARG;
+    }
+    void* base = nullptr;
+    const int rc = noalloc_scratch(c->opt.device, &base);
+    if (rc != AMMSB_OK) return rc;
+    d_err = (int32_t*)base;
+    scr = (double*)((char*)base + kNoAllocHeader);
+  } else {
+    const size_t need = sizeof(double) * (size_t)(body + 2);
+    if (need > c->ll_scratch_bytes) {
+      if (c->d_ll_scratch) HIP_TRY(hipFree(c->d_ll_scratch));
+      c->d_ll_scratch = nullptr;
+      c->ll_scratch_bytes = 0;
+      HIP_TRY(hipMalloc((void**)&c->d_ll_scratch, need));
+      c->ll_scratch_bytes = need;
+    }
+    scr = c->d_ll_scratch;
+    d_err = (int32_t*)(c->d_ll_scratch + body + 1);
   }
-  int32_t* d_err = (int32_t*)(c->d_ll_scratch + body + 1);
   HIP_TRY(launch_fill_u32((uint32_t*)d_err, 0u, 1, s));  // a kernel, not a memset node
   LogLikArgs a{};
   a.pi = c->d_pi;
@@ -1605,10 +1668,10 @@ int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi
   a.pair_nbr = pair_nbr;
   a.pair_y = pair_y;
   a.epsilon = c->hp.epsilon;
-  a.per_source = per_source ? per_source : c->d_ll_scratch;
+  a.per_source = per_source ? per_source : scr;
   a.gout = gout;
   a.grad_phi = grad_phi;
-  a.dbeta_partials = c->d_ll_scratch;
+  a.dbeta_partials = scr;
   a.blocks = blocks;
   a.N = c->N;
   a.P = P;
@@ -1616,7 +1679,7 @@ int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi
   if (!bwd) {
     HIP_TRY(launch_loglik(a, c->prec, false, s, c->ll_pairs[0], c->ll_prefetch[0],
                           c->ll_fwd_rows != 0));
-    double* out = total ? total : c->d_ll_scratch + body;
+    double* out = total ? total : scr + body;
     HIP_TRY(launch_sum(a.per_source, M, out, s));
   } else {
     if (!grad_phi) {
@@ -1625,8 +1688,7 @@ int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi
     }
     HIP_TRY(launch_loglik(a, c->prec, true, s, c->ll_pairs[1], c->ll_prefetch[1], rows_bwd));
     if (grad_theta)
-      HIP_TRY(launch_theta_grad(c->d_ll_scratch, (int32_t)prows, c->K, a.theta, gout, grad_theta,
-                                s));
+      HIP_TRY(launch_theta_grad(scr, (int32_t)prows, c->K, a.theta, gout, grad_theta, s));
   }
   int32_t herr = 0;
   HIP_TRY(hipMemcpyAsync(&herr, d_err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1653,6 +1715,15 @@ int ammsb_loglik_bwd(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* 
                      void* grad_phi, double* grad_theta, void* stream) {
   return loglik_common(c, M, nodes, phi, phi_ld, theta, pair_ptr, pair_nbr, pair_y, P, true,
                        nullptr, nullptr, gout, grad_phi, grad_theta, stream);
+}
+
+int ammsb_loglik_bwd_noalloc(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi,
+                             int64_t phi_ld, const double* theta, const int64_t* pair_ptr,
+                             const int32_t* pair_nbr, const uint8_t* pair_y, int64_t P,
+                             const double* gout, void* grad_phi, double* grad_theta,
+                             void* stream) {
+  return loglik_common(c, M, nodes, phi, phi_ld, theta, pair_ptr, pair_nbr, pair_y, P, true,
+                       nullptr, nullptr, gout, grad_phi, grad_theta, stream, true);
 }
 
 int ammsb_device_ptrs(ammsb_ctx* c, void** pi, void** theta, void** beta) {
