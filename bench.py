@@ -2,26 +2,48 @@
 """Benchmark of the SGRLD a-MMSB hot path on one MI355X.
 
 Metric: update_phi node*K updates/s -- nodes whose phi/pi rows are updated,
-times K, per second of whole SGRLD iterations (minibatch and neighbour
-sampling, update_phi, update_pi, update_beta, and the held-out perplexity
-at the reference's interval of 50 steps), with graph and state resident in
-HBM before the timed region starts.
+times K, per second of whole SGRLD iterations -- together with the
+time-to-target held-out perplexity of the FP32 precision mode against the
+FP64 path.  Graph and state are resident in HBM before any timed region.
 
-Default workload (one GPU): N=1,000,000, K=256, FP64, community graph with
-mean degree 50 (~25M edges), held-out ratio 0.1, alpha=1/K, eta=(1,1),
-epsilon=1e-7, a=0.01, b=1024, c=0.55, stratified random node minibatches of
-4,096 nodes (m=4095 non-link pairs; link minibatches gather sources until
-the next one would exceed m), 128 neighbour samples (129 drawn per node).
+Default workload (one GPU): N = 4,000,000, K = 1024, community graph with
+mean degree 40 (about 80M edges), held-out ratio 0.1, alpha = 1/K,
+eta = (1, 1), epsilon = 1e-7, a = 0.01, b = 1024, c = 0.55, stratified
+random node minibatches of 4,096 nodes (m = 4,095 non-link pairs; link
+minibatches gather sources until the next would exceed m,
+sampler.max-source = 1024), 128 neighbour samples (129 drawn per node),
+perplexity every 50 steps.  `value` is the FP32 precision mode; the FP64
+path runs the same measurement beside it (`fp64`).
 
-The JSON line also carries
-  roofline      achieved HBM GB/s of the update_phi kernel (algorithmic
-                bytes per launch / its average launch time measured with HIP
-                events on the library's stream over the timed region)
-  cpu_baseline  the CPU oracle (C++ restatement of the reference loop) on
-                the host cores, on a bounded sample of the same workload
+Per precision, from the same initial state and Philox seed:
+  * time to target: the reference's loop (perplexity before the first step
+    and every 50 steps, mcmc_sampler_stochastic.cc:112-130) for
+    --ttt-steps steps; the target is the FP64 run's last perplexity, reached
+    at the first checkpoint within 1 % of it;
+  * throughput: --warmup untimed steps, then exactly --steps timed steps.
+    The sample-ahead ring draws `ring_slots` iterations per refill and the
+    perplexity runs every 50 steps, so a short window holds more or fewer
+    of them than a long run.  The window starts with a refill; the refill
+    and one perplexity evaluation are timed in the same run, and
+        ms_per_step = (window - refills_in_window * refill) / steps
+                      + refill / ring_slots + perplexity / 50,
+    the per-step cost of a long run.  The raw window figures are reported
+    too (`window`).
+The JSON line carries
+  roofline      the update_phi kernel's algorithmic bytes per launch / its
+                average launch time (HIP events on the library's stream over
+                the timed window), against the 8 TB/s HBM peak
+  time_to_target, errors_vs_fp64   FP32 against FP64 from identical seeds
+  cpu_baseline  the oracle (C++ restatement of the reference loop) on the
+                host cores, on the same workload and state: median over 20
+                iterations after 5 warm-up, threaded as the distributed
+                reference's OpenMP loops, FP32 (`value`) and FP64; and one
+                thread (the reference's sequential build) on a shorter
+                sample
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -32,28 +54,37 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+INTERVAL = 50  # perplexity interval (mcmc_sampler_stochastic.cc:17-26)
 
 WORKLOADS = {
-    # name: (N, K, mean_degree, n, m, precision)
-    "n1m-k256-fp64": dict(N=1_000_000, K=256, deg=50.0, n=128, m=4095, precision="fp64",
-               desc="N=1M K=256 FP64 mean-degree-50 community graph, 4096-node minibatch, "
-                    "128 neighbour samples"),
-    "n4m-k1024-fp32": dict(N=4_000_000, K=1024, deg=40.0, n=128, m=4095, precision="fp32",
-               desc="N=4M K=1024 FP32 precision mode, mean-degree-40 community graph, "
-                    "4096-node minibatch, 128 neighbour samples"),
-    "n4m-k1024-fp64": dict(N=4_000_000, K=1024, deg=40.0, n=128, m=4095, precision="fp64",
-                    desc="N=4M K=1024 FP64, mean-degree-40 community graph, 4096-node "
-                         "minibatch, 128 neighbour samples"),
-    "n5000-k32-fp64": dict(N=5000, K=32, deg=None, n=32, m=500, precision="fp64",
-               desc="N=5000 K=32 FP64 planted graph, m=500, 32 neighbour samples"),
+    "n4m-k1024": dict(N=4_000_000, K=1024, deg=40.0, n=128, m=4095, primary="fp32",
+                      desc="N=4M K=1024 mean-degree-40 community graph, 4096-node minibatches, "
+                           "128 neighbour samples; FP32 precision mode (value) and FP64 beside it"),
+    "n1m-k256": dict(N=1_000_000, K=256, deg=50.0, n=128, m=4095, primary="fp64",
+                     desc="N=1M K=256 mean-degree-50 community graph, 4096-node minibatches, "
+                          "128 neighbour samples; FP64 (value) and FP32 beside it"),
+    "n5000-k32": dict(N=5000, K=32, deg=None, n=32, m=500, primary="fp64",
+                      desc="N=5000 K=32 planted graph, m=500, 32 neighbour samples; FP64 (value) "
+                           "and FP32 beside it"),
 }
+# one-precision names: (workload, primary precision)
+ALIASES = {"n4m-k1024-fp32": ("n4m-k1024", "fp32"), "n4m-k1024-fp64": ("n4m-k1024", "fp64"),
+           "n1m-k256-fp64": ("n1m-k256", "fp64"), "n1m-k256-fp32": ("n1m-k256", "fp32"),
+           "n5000-k32-fp64": ("n5000-k32", "fp64")}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(w, seed):
+def resolve_workload(name):
+    if name in ALIASES:
+        wname, prec = ALIASES[name]
+        return dict(WORKLOADS[wname], name=wname), prec
+    return dict(WORKLOADS[name], name=name), WORKLOADS[name]["primary"]
+
+
+def build_graph(w, seed):
     import mcmc_for_ammsb_amd as M
     t = time.time()
     if w["deg"] is None:
@@ -63,44 +94,15 @@ def build_workload(w, seed):
     s = M.make_split(g, 0.1, 42)
     log(f"graph N={g.N} E={g.num_edges} held-out={len(s.held_out_labels)} "
         f"({time.time() - t:.1f}s)")
-    opts = M.Options(K=w["K"], num_node_sample=w["n"], mini_batch_size=w["m"],
+    return g, s
+
+
+def make_options(w, precision):
+    import mcmc_for_ammsb_amd as M
+    return M.Options(K=w["K"], num_node_sample=w["n"], mini_batch_size=w["m"],
                      sampler_max_source=1024 if w["deg"] is not None else 1,
                      a=0.01, b=1024.0, c=0.55, epsilon=1e-7, eta0=1.0, eta1=1.0,
-                     interval=50, random_seed=42, precision=w["precision"])
-    return g, s, opts
-
-
-def cpu_baseline(g, s, opts, pi, theta, start_step, budget_s, threads):
-    """The oracle loop on the host cores (bounded sample, no perplexity)."""
-    from tests.oracle_lib import Oracle
-    K = opts.K
-    orc = Oracle(g, s, K, precision=opts.precision, num_node_sample=opts.num_node_sample,
-                 a=opts.a, b=opts.b, c=opts.c, epsilon=opts.epsilon, eta0=opts.eta0,
-                 eta1=opts.eta1)
-    orc.set_state(pi, theta)
-    step = start_step if (start_step - 1) % 50 != 0 else start_step + 1
-    orc.set_counters(step, 2)
-    done, nodes, secs, phi_secs = 0, 0.0, 0.0, 0.0
-    t0 = time.time()
-    while time.time() - t0 < budget_s and done < 200:
-        ta = time.time()
-        _, ph = orc.run(1, 0, opts.mini_batch_size, opts.sampler_max_source,
-                        10 ** 9, 42, threads=threads)
-        secs += time.time() - ta
-        phi_secs += ph[3]
-        nodes += ph[5]
-        done += 1
-    return {
-        "value": nodes * K / secs,
-        "unit": "node*K/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"{done} full iterations (samplers, update_phi+pi, update_beta) of the same "
-                  f"workload from the GPU run's state at step {step}, C++ oracle with "
-                  f"{threads} threads for the node loop",
-        "update_phi_node_k_per_s": nodes * K / phi_secs if phi_secs > 0 else None,
-        "seconds": secs,
-    }
+                     interval=INTERVAL, random_seed=42, precision=precision)
 
 
 def host_info():
@@ -122,17 +124,19 @@ def host_info():
 
 
 def trajectory(smp, steps, interval):
-    """Checkpoints (step, wall s including perplexity, wall s excluding it, perplexity),
-    perplexity before the first step and then every `interval` steps
-    (mcmc_sampler_stochastic.cc:112-130)."""
+    """The reference loop's perplexity schedule (mcmc_sampler_stochastic.cc:
+    112-130): perplexity before the first step and after every `interval`
+    steps.  Checkpoints (step, wall s including perplexity, wall s excluding
+    it, perplexity)."""
     out = []
     t_all, t_ppx = 0.0, 0.0
     done = 0
     while True:
         a = time.perf_counter()
         p = smp.perplexity()
-        t_ppx += time.perf_counter() - a
-        t_all += time.perf_counter() - a
+        d = time.perf_counter() - a
+        t_ppx += d
+        t_all += d
         out.append((done + 1, t_all, t_all - t_ppx, p))
         if done >= steps:
             break
@@ -143,71 +147,302 @@ def trajectory(smp, steps, interval):
     return out
 
 
-def run_ttt(args):
-    """Time-to-target of the FP32 precision mode against the FP64 path on the
-    same workload, initial state and Philox seed; error per quantity."""
-    import mcmc_for_ammsb_amd as M
-    w = dict(WORKLOADS[args.workload])
-    w["precision"] = "fp64"
-    g, s, opts = build_workload(w, args.seed)
+def tuning(smp, name):
+    import ctypes
+    from mcmc_for_ammsb_amd import _lib
+    v = ctypes.c_int64()
+    _lib.check(_lib.lib.ammsb_get_tuning(smp._h, name.encode(), ctypes.byref(v)))
+    return v.value
+
+
+def get_beta(smp):
+    import ctypes
+    from mcmc_for_ammsb_amd import _lib
+    beta = np.empty(smp.info["K"])
+    _lib.check(_lib.lib.ammsb_get_state(smp._h, None, None, _lib.ptr(beta, ctypes.c_double)))
+    return beta
+
+
+def measure(smp, args, sample_nodes):
+    """Time to target and throughput of one context (see the module
+    docstring).  Returns a dict; the context is left after the last step."""
+    inf = smp.info
+    K, nnb = inf["K"], inf["num_node_sample"] + 1
+    fsize = 8 if inf["precision"] == 0 else 4
+    seed = smp.options.random_seed
     res = {}
-    states = {}
-    sample_nodes = np.sort(np.random.default_rng(0).choice(g.N, size=min(g.N, 65536), replace=False))
-    for prec in ("fp64", "fp32"):
-        o = M.Options(**{**opts.__dict__, "precision": prec, "remains": []})
-        smp = M.MCMCSamplerStochastic(o, g, s)
-        smp.init(o.random_seed)
-        smp.step(2)  # device warm-up on the same draws as the run (state reset below)
-        smp.init(o.random_seed)
-        smp.set_counters(1, 1)
-        traj = trajectory(smp, args.ttt_steps, args.ttt_interval)
-        rows = smp.get_rows(sample_nodes).astype(np.float64)  # 65,536 random rows
-        _, _, beta = smp.get_state() if g.N * (opts.K + 1) <= 50_000_000 else (None, None, None)
-        if beta is None:
-            beta = np.empty(opts.K)
-            import ctypes
-            from mcmc_for_ammsb_amd import _lib
-            _lib.check(_lib.lib.ammsb_get_state(smp._h, None, None, _lib.ptr(beta, ctypes.c_double)))
-        states[prec] = (rows, beta)
-        res[prec] = traj
-        log(f"{prec}: " + " ".join(f"{t}:{p:.5f}@{a:.2f}s" for t, a, _, p in traj))
-        smp.close()
-    target = res["fp64"][-1][3]
-    def reach(traj):
-        for t, a, b, p in traj:
-            if abs(p - target) <= 0.01 * target or p <= target:
-                return t, a, b
-        return None, None, None
-    t64 = reach(res["fp64"])
-    t32 = reach(res["fp32"])
-    pi64, b64 = states["fp64"]
-    pi32, b32 = states["fp32"]
-    K = pi64.shape[1] - 1
-    dpi = np.abs(pi32[:, :K] - pi64[:, :K])
-    rel_ppx = [abs(a[3] - b[3]) / b[3] for a, b in zip(res["fp32"], res["fp64"])]
-    out = {
-        "metric": "time-to-target held-out perplexity, FP32 precision mode vs FP64",
-        "workload": f"{args.workload}: {w['desc']} (FP32 and FP64 from the same state and seed)",
-        "steps": args.ttt_steps, "interval": args.ttt_interval,
-        "target_perplexity": target,
-        "fp64_time_to_target_s": t64[1], "fp64_time_to_target_s_excl_perplexity": t64[2],
-        "fp64_steps_to_target": t64[0],
-        "fp32_time_to_target_s": t32[1], "fp32_time_to_target_s_excl_perplexity": t32[2],
-        "fp32_steps_to_target": t32[0],
-        "speedup": (t64[1] / t32[1]) if t32[1] else None,
-        "speedup_excl_perplexity": (t64[2] / t32[2]) if t32[2] else None,
-        "errors_vs_fp64": {
-            "rows_compared": int(len(sample_nodes)),
-            "pi_max_abs": float(dpi.max()), "pi_mean_abs": float(dpi.mean()),
-            "pi_max_rel": float((dpi / np.maximum(pi64[:, :K], 1e-30)).max()),
-            "phi_sum_max_rel": float(np.abs(pi32[:, K] / pi64[:, K] - 1).max()),
-            "beta_max_rel": float(np.abs(b32 / b64 - 1).max()),
-            "beta_mean_rel": float(np.abs(b32 / b64 - 1).mean()),
-            "perplexity_max_rel": float(max(rel_ppx)), "perplexity_mean_rel": float(np.mean(rel_ppx)),
-            "perplexity_final_rel": rel_ppx[-1]},
-        "fp64_checkpoints": res["fp64"], "fp32_checkpoints": res["fp32"],
+    # device warm-up on the run's own draws, then the initial state again
+    smp.init(seed)
+    smp.step(2)
+    smp.init(seed)
+    smp.set_counters(1, 1)
+    if args.ttt_steps > 0:
+        res["trajectory"] = trajectory(smp, args.ttt_steps, INTERVAL)
+        res["rows"] = smp.get_rows(sample_nodes).astype(np.float64)
+        res["beta"] = get_beta(smp)
+    # ---- throughput
+    smp.step(args.warmup, sync=True)
+    slots = tuning(smp, "ring_slots")
+    step0, avg0 = smp.info["step_count"], smp.info["average_count"]
+    smp.enable_timers(2)  # HIP events around update_phi only
+    tb = smp.timers()
+    n0 = smp.totals()[0]
+    smp.set_counters(step0, avg0)  # the window starts with a ring refill
+    t0 = time.perf_counter()
+    smp.step(args.steps, sync=True)  # ends with a stream synchronisation
+    t_window = time.perf_counter() - t0
+    n_window = smp.totals()[0] - n0
+    ta = smp.timers()
+    smp.enable_timers(False)
+    phi_s = ta["update_phi"][0] - tb["update_phi"][0]
+    phi_n = ta["update_phi"][1] - tb["update_phi"][1]
+    refills_in_window = math.ceil(args.steps / slots)
+    # the refill: two ring periods with every phase timed, starting at a refill
+    smp.enable_timers(True)
+    tb = smp.timers()
+    st = smp.info
+    smp.set_counters(st["step_count"], st["average_count"])
+    smp.step(2 * slots, sync=True)
+    ta = smp.timers()
+    smp.enable_timers(False)
+    phases = {k: (ta[k][0] - tb[k][0], ta[k][1] - tb[k][1]) for k in ta}
+    fills = max(phases["sample_mini_batch"][1], 1)
+    t_fill = (phases["sample_mini_batch"][0] + phases["sample_neighbor_nodes"][0]) / fills
+    # the held-out perplexity: median of three evaluations
+    ppx_times = []
+    for _ in range(3):
+        a = time.perf_counter()
+        smp.perplexity()
+        ppx_times.append(time.perf_counter() - a)
+    t_ppx = float(np.median(ppx_times))
+    # a contiguous read of the resident pi array: the box's streaming read
+    # rate, and a read of known size for calibrating HBM counters
+    probe_s, probe_bytes = smp.probe_read(5)
+    t_iter = (t_window - refills_in_window * t_fill) / args.steps
+    sec_per_step = t_iter + t_fill / slots + t_ppx / INTERVAL
+    nodes_per_step = n_window / args.steps
+    avg_launch = phi_s / max(phi_n, 1)
+    nodes_per_launch = n_window / max(phi_n, 1)
+    # algorithmic bytes of update_phi per node: n_nb neighbour rows + own row
+    # read + own row written (K each), the n_nb neighbour ids and phi_sum
+    bytes_per_node = (nnb + 2) * K * fsize + nnb * 4 + 2 * fsize
+    achieved = nodes_per_launch * bytes_per_node / avg_launch / 1e9
+    res.update({
+        "value": nodes_per_step * K / sec_per_step,
+        "ms_per_step": 1e3 * sec_per_step,
+        "nodes_per_step": nodes_per_step,
+        "components_ms": {"iteration": 1e3 * t_iter, "ring_refill": 1e3 * t_fill,
+                          "ring_slots": slots, "perplexity": 1e3 * t_ppx,
+                          "perplexity_interval": INTERVAL},
+        "window": {"steps": args.steps, "ms_per_step": 1e3 * t_window / args.steps,
+                   "ring_refills": refills_in_window, "perplexity_evaluations": 0},
+        "update_phi": {"avg_launch_us": 1e6 * avg_launch, "launches": phi_n,
+                       "nodes_per_launch": nodes_per_launch,
+                       "algorithmic_bytes_per_launch": nodes_per_launch * bytes_per_node,
+                       "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS,
+                       "kernel_node_k_per_s": n_window * K / phi_s if phi_s > 0 else None},
+        "stream_read_GBs": probe_bytes / probe_s / 1e9,
+        "phase_us_per_step": {k: 1e6 * sec / (2 * slots) for k, (sec, n) in phases.items()},
+        "phase_ticks": {k: n for k, (sec, n) in phases.items()},
+    })
+    return res
+
+
+def reach(traj, target):
+    for t, a, b, p in traj:
+        if abs(p - target) <= 0.01 * target or p <= target:
+            return t, a, b
+    return None, None, None
+
+
+def ttt_summary(r64, r32, K):
+    """FP32 against FP64 from the same state and seed: time to the FP64 run's
+    final perplexity (within 1 %), and the error of FP32 per quantity."""
+    t64, t32 = r64["trajectory"], r32["trajectory"]
+    target = t64[-1][3]
+    a64, a32 = reach(t64, target), reach(t32, target)
+    dpi = np.abs(r32["rows"][:, :K] - r64["rows"][:, :K])
+    rb = np.abs(r32["beta"] / r64["beta"] - 1)
+    rp = [abs(a[3] - b[3]) / b[3] for a, b in zip(t32, t64)]
+    return {
+        "target_perplexity": target, "steps": t64[-1][0] - 1, "interval": INTERVAL,
+        "fp64_s": a64[1], "fp64_s_excl_perplexity": a64[2], "fp64_steps": a64[0],
+        "fp32_s": a32[1], "fp32_s_excl_perplexity": a32[2], "fp32_steps": a32[0],
+        "speedup": a64[1] / a32[1] if a32[1] else None,
+        "speedup_excl_perplexity": a64[2] / a32[2] if a32[2] else None,
+    }, {
+        "against": "FP64 GPU run from the same state and Philox seed, after the same steps",
+        "rows_compared": int(len(dpi)),
+        "pi_max_abs": float(dpi.max()), "pi_mean_abs": float(dpi.mean()),
+        "phi_sum_max_rel": float(np.abs(r32["rows"][:, K] / r64["rows"][:, K] - 1).max()),
+        "beta_max_rel": float(rb.max()), "beta_mean_rel": float(rb.mean()),
+        "perplexity_max_rel": float(max(rp)), "perplexity_mean_rel": float(np.mean(rp)),
+        "perplexity_final_rel": float(rp[-1]),
     }
-    print(json.dumps(out), flush=True)
+
+
+def cpu_leg(orc, opts, start_step, threads, warmup, timed, with_ppx):
+    """Oracle iterations on the host: `warmup` untimed, then `timed` timed
+    one by one; with_ppx: the first warm-up iteration falls on a perplexity
+    step and its evaluation is timed.  Median iteration, amortised
+    perplexity."""
+    K = opts.K
+    step = start_step
+    if with_ppx:
+        step = (step - 1) // INTERVAL * INTERVAL + 1 + INTERVAL  # (step - 1) % 50 == 0
+    else:
+        while (step - 1) % INTERVAL == 0 or (step - 1) // INTERVAL != (step + warmup + timed - 2) // INTERVAL:
+            step += 1
+    orc.set_counters(step, 2)
+    t_ppx, it_s, phi_s, nodes = None, [], [], []
+    for i in range(warmup + timed):
+        _, ph = orc.run(1, 0, opts.mini_batch_size, opts.sampler_max_source, INTERVAL, 42,
+                        threads=threads)
+        if i == 0 and with_ppx:
+            t_ppx = float(ph[0])
+        if i >= warmup:
+            it_s.append(float(ph[1] + ph[2] + ph[3] + ph[4]))
+            phi_s.append(float(ph[3]))
+            nodes.append(float(ph[5]))
+    t_it = float(np.median(it_s))
+    sec = t_it + (t_ppx / INTERVAL if t_ppx is not None else 0.0)
+    return {
+        "value": float(np.median(nodes)) * K / sec, "unit": "node*K/s", "cores": threads,
+        "median_iteration_s": t_it, "perplexity_s": t_ppx,
+        "update_phi_node_k_per_s": float(np.median(nodes)) * K / float(np.median(phi_s)),
+        "sample": f"{warmup} warm-up + {timed} timed full iterations (samplers, update_phi+pi, "
+                  f"update_beta) of the same workload from the GPU run's final state, from step "
+                  f"{step}; median iteration"
+                  + (", plus one perplexity evaluation amortised over 50 steps" if with_ppx
+                     else "; perplexity not included"),
+    }
+
+
+def cpu_baseline(g, s, w, pi32, theta, start_step, args):
+    """The oracle loop on the host cores, FP32 and FP64, threaded (the
+    distributed reference's OpenMP loops) and on one thread."""
+    from tests.oracle_lib import Oracle
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    out = {}
+    for prec in ("fp32", "fp64"):
+        opts = make_options(w, prec)
+        t = time.time()
+        orc = Oracle(g, s, opts.K, precision=prec, num_node_sample=opts.num_node_sample,
+                     a=opts.a, b=opts.b, c=opts.c, epsilon=opts.epsilon, eta0=opts.eta0,
+                     eta1=opts.eta1)
+        orc.set_state(pi32 if prec == "fp32" else pi32.astype(np.float64), theta)
+        log(f"cpu baseline {prec}: oracle ready ({time.time() - t:.1f}s)")
+        t = time.time()
+        omp = cpu_leg(orc, opts, start_step, threads, args.cpu_warmup, args.cpu_iters, True)
+        log(f"  {threads} threads: {omp['value']:.3e} node*K/s ({time.time() - t:.1f}s)")
+        t = time.time()
+        seq = cpu_leg(orc, opts, start_step, 1, 1, args.cpu_seq_iters, False)
+        log(f"  1 thread: {seq['value']:.3e} node*K/s ({time.time() - t:.1f}s)")
+        out[prec] = {"threads": omp, "one_thread": seq}
+        del orc
+    cb = dict(out["fp32"]["threads"])
+    cb.update({"kind": "port", "precision": "fp32",
+               "fp64": out["fp64"]["threads"],
+               "one_thread": {"fp32": out["fp32"]["one_thread"],
+                              "fp64": out["fp64"]["one_thread"]},
+               "host": host_info()})
+    return cb
+
+
+def run_throughput(args, ttt_only=False):
+    import mcmc_for_ammsb_amd as M
+    w, primary = resolve_workload(args.workload)
+    g, s = build_graph(w, args.seed)
+    sample_nodes = np.sort(np.random.default_rng(0).choice(g.N, size=min(g.N, 65536),
+                                                           replace=False))
+    order = ("fp64", "fp32")
+    res, keep = {}, None
+    for prec in order:
+        opts = make_options(w, prec)
+        smp = M.MCMCSamplerStochastic(opts, g, s)
+        t = time.time()
+        res[prec] = measure(smp, args, sample_nodes)
+        r = res[prec]
+        log(f"{prec}: {r['ms_per_step']:.4f} ms/step amortised ({r['window']['ms_per_step']:.4f} "
+            f"in the window), {r['value']:.3e} node*K/s, update_phi "
+            f"{r['update_phi']['avg_launch_us']:.1f} us = {r['update_phi']['achieved_GBs']:.0f} GB/s "
+            f"({time.time() - t:.1f}s)")
+        if "trajectory" in r:
+            log(f"  trajectory: " + " ".join(f"{c[0]}:{c[3]:.5f}@{c[1]:.2f}s"
+                                            for c in r["trajectory"][::10]))
+        if prec == primary and not args.no_cpu_baseline and not ttt_only:
+            pi, theta, _ = smp.get_state()
+            keep = (pi if pi.dtype == np.float32 else pi.astype(np.float32), theta,
+                    smp.info["step_count"])
+            del pi
+        smp.close()
+    ttt, errs = (ttt_summary(res["fp64"], res["fp32"], w["K"])
+                 if args.ttt_steps > 0 else (None, None))
+    if ttt:
+        log(f"time to target {ttt['target_perplexity']:.6f}: FP64 {ttt['fp64_s']:.3f}s "
+            f"({ttt['fp64_steps']} steps), FP32 {ttt['fp32_s']:.3f}s ({ttt['fp32_steps']} steps), "
+            f"speedup {ttt['speedup']:.2f}")
+    if ttt_only:
+        print(json.dumps({"metric": "time-to-target held-out perplexity, FP32 vs FP64",
+                          "workload": f"{w['name']}: {w['desc']}", "time_to_target": ttt,
+                          "errors_vs_fp64": errs,
+                          "fp64_checkpoints": res["fp64"].get("trajectory"),
+                          "fp32_checkpoints": res["fp32"].get("trajectory")}), flush=True)
+        return 0
+    p = res[primary]
+    other = "fp64" if primary == "fp32" else "fp32"
+    up = p["update_phi"]
+    traffic = None
+    try:
+        with open(args.traffic) as f:
+            tr = json.load(f)
+        entry = tr.get(f"{w['name']}-{primary}")
+        if isinstance(entry, dict):
+            traffic = entry.get("bytes_per_launch")
+    except (OSError, ValueError):
+        traffic = None
+    out = {
+        "metric": "update_phi node*K updates/s",
+        "value": p["value"],
+        "unit": "node*K/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": p["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if primary == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{w['name']}: {w['desc']}", "N": g.N, "K": w["K"],
+                   "edges": g.num_edges, "num_node_sample": w["n"], "mini_batch_size": w["m"],
+                   "sampler_max_source": 1024 if w["deg"] is not None else 1,
+                   "perplexity_interval": INTERVAL, "held_out": int(len(s.held_out_labels)),
+                   "nodes_per_step": p["nodes_per_step"], "precision": primary,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": up["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": up["frac"], "traffic": traffic,
+                     "kernel": "k_update_phi", "avg_launch_us": up["avg_launch_us"],
+                     "algorithmic_bytes_per_launch": up["algorithmic_bytes_per_launch"],
+                     "nodes_per_launch": up["nodes_per_launch"]},
+        "components_ms": p["components_ms"],
+        "window": p["window"],
+        "stream_read_GBs": p["stream_read_GBs"],
+        other: {k: res[other][k] for k in ("value", "ms_per_step", "components_ms", "window",
+                                           "update_phi", "stream_read_GBs")},
+        "time_to_target": ttt,
+        "errors_vs_fp64": errs,
+        "phase_us_per_step": p["phase_us_per_step"],
+        "perplexity_last": p["trajectory"][-1][3] if "trajectory" in p else None,
+        "cpu_baseline": None,
+    }
+    rank = int(os.environ.get("RANK", "0"))
+    if rank == 0 and keep is not None:
+        pi32, theta, step = keep
+        out["cpu_baseline"] = cpu_baseline(g, s, w, pi32, theta, step, args)
+        del pi32
+    if rank == 0:
+        print(json.dumps(out), flush=True)
     return 0
 
 
@@ -370,8 +605,9 @@ def run_operator(args):
     checks its ids on the host and allocates its device copies.  Reported
     next to the resident kernel rate; never the headline value."""
     import mcmc_for_ammsb_amd as M
-    w = WORKLOADS[args.workload]
-    g, s, opts = build_workload(w, args.seed)
+    w, prec = resolve_workload(args.workload)
+    g, s = build_graph(w, args.seed)
+    opts = make_options(w, prec)
     smp = M.MCMCSamplerStochastic(opts, g, s)
     smp.init(opts.random_seed)
     inf = smp.info
@@ -485,21 +721,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="n1m-k256-fp64", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="n4m-k1024", choices=sorted(WORKLOADS) + sorted(ALIASES))
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--ttt-steps", type=int, default=3000,
+                    help="steps of the time-to-target trajectory (0: none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--phi-waves", type=int, default=0, help="update_phi wavefronts per node")
-    ap.add_argument("--phi-unroll", type=int, default=0, help="update_phi rows in flight")
-    ap.add_argument("--timer-mode", default="phi", choices=["phi", "all", "off"],
-                    help="HIP events in the timed region: update_phi only, every phase, none")
-    ap.add_argument("--async-ring", type=int, default=-1,
-                    help="1: samplers draw ahead on a second stream; 0: on the main stream")
-    ap.add_argument("--phi-split", type=int, default=-1,
-                    help="1: update_phi gather + finish kernels; 0: fused step + scatter")
+    ap.add_argument("--cpu-warmup", type=int, default=5)
+    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--cpu-seq-iters", type=int, default=3)
     ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "update_phi_traffic.json"))
-    ap.add_argument("--mode", default="throughput", choices=["throughput", "ttt", "diff", "batch", "operator"])
+    ap.add_argument("--mode", default="throughput",
+                    choices=["throughput", "ttt", "diff", "batch", "operator"])
     ap.add_argument("--chains", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-separate", action="store_true")
@@ -508,144 +741,23 @@ def main():
                     help="diff mode: 1 whole rows per wavefront, 0 the K-chunked forward")
     ap.add_argument("--ll-bwd-rows", type=int, default=-1,
                     help="diff mode: 1 whole rows per wavefront, 0 the K-chunked adjoint")
-    ap.add_argument("--ttt-steps", type=int, default=3000)
-    ap.add_argument("--ttt-interval", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         log("bench.py measures one GPU: the library has no multi-GPU path "
             "(one MI355X per process); run with --gpus 1")
         return 2
-    rank = int(os.environ.get("RANK", "0"))
+    if args.steps < 1 or args.warmup < 0:
+        log("--steps must be >= 1 and --warmup >= 0")
+        return 2
     if args.mode == "ttt":
-        return run_ttt(args)
+        return run_throughput(args, ttt_only=True)
     if args.mode == "diff":
         return run_diff(args)
     if args.mode == "batch":
         return run_batch(args)
     if args.mode == "operator":
         return run_operator(args)
-
-    import mcmc_for_ammsb_amd as M
-    w = WORKLOADS[args.workload]
-    g, s, opts = build_workload(w, args.seed)
-    smp = M.MCMCSamplerStochastic(opts, g, s)
-    if args.phi_waves or args.phi_unroll:
-        cur = smp.get_tuning()
-        smp.set_tuning(phi_waves=args.phi_waves or cur["phi_waves"],
-                       phi_unroll=args.phi_unroll or cur["phi_unroll"])
-    if args.phi_split >= 0:
-        smp.set_tuning(phi_split=args.phi_split)
-    if args.async_ring >= 0:
-        smp.set_tuning(async_ring=args.async_ring)
-    inf = smp.info
-    smp.init(opts.random_seed)
-    K, nnb = inf["K"], inf["num_node_sample"] + 1
-    fsize = 8 if opts.precision == "fp64" else 4
-
-    smp.run(max_steps=args.warmup, print_log=False)
-    # timed region: HIP events around update_phi only (for the roofline);
-    # the per-phase breakdown comes from a separate pass below, because an
-    # event pair around every phase adds gaps to the stream
-    smp.enable_timers(2 if args.timer_mode == "phi" else (1 if args.timer_mode == "all" else 0))
-    t_before = smp.timers()
-    n0, e0 = smp.totals()
-    t0 = time.perf_counter()
-    ckpt = smp.run(max_steps=args.steps, print_log=False)  # ends with a stream sync
-    t1 = time.perf_counter()
-    n1, e1 = smp.totals()
-    t_after = smp.timers()
-    smp.enable_timers(False)
-    elapsed = t1 - t0
-    nodes = n1 - n0
-    value = nodes * K / elapsed
-
-    ph = {k: (t_after[k][0] - t_before[k][0], t_after[k][1] - t_before[k][1]) for k in t_after}
-    bd_steps = min(args.steps, 100)
-    smp.enable_timers(True)
-    tb0 = smp.timers()
-    nb0 = smp.totals()[0]
-    smp.run(max_steps=bd_steps, print_log=False)
-    nb1 = smp.totals()[0]
-    tb1 = smp.timers()
-    smp.enable_timers(False)
-    ph_all = {k: (tb1[k][0] - tb0[k][0], tb1[k][1] - tb0[k][1]) for k in tb1}
-    phi_nodes = nodes
-    if args.timer_mode == "off":
-        ph, phi_nodes = ph_all, nb1 - nb0  # roofline from the breakdown pass
-    phi_s, phi_n = ph["update_phi"]
-    avg_launch = phi_s / max(phi_n, 1)
-    nodes_per_launch = phi_nodes / max(phi_n, 1)
-    # algorithmic bytes of one (node, k) update: n_nb neighbour rows + own row
-    # read + own row written; per node the n_nb neighbour ids and phi_sum
-    bytes_per_node = (nnb + 2) * K * fsize + nnb * 4 + 2 * fsize
-    bytes_per_launch = nodes_per_launch * bytes_per_node
-    achieved = bytes_per_launch / avg_launch / 1e9
-    traffic = None
-    if os.path.exists(args.traffic):
-        try:
-            with open(args.traffic) as f:
-                tr = json.load(f)
-            entry = tr.get(args.workload)
-            if isinstance(entry, dict):
-                traffic = entry.get("bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
-
-    probe_s, probe_bytes = smp.probe_read(5)  # contiguous read of the resident state
-    log(f"contiguous read of {probe_bytes / 1e9:.2f} GB: {probe_bytes / probe_s / 1e9:.0f} GB/s")
-    log(f"phase breakdown ({bd_steps} further steps, every phase timed)")
-    log("phase            total_s   ticks   per_tick_us")
-    for k, (sec, n) in ph_all.items():
-        log(f"{k:<16} {sec:9.4f} {n:7d} {1e6 * sec / max(n, 1):12.2f}")
-    log(f"checkpoints (step, perplexity): {ckpt}")
-
-    out = {
-        "metric": "update_phi node*K updates/s",
-        "value": value,
-        "unit": "node*K/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if opts.precision == "fp64" else "f32",
-        "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {w['desc']}", "N": g.N, "K": K,
-                   "edges": g.num_edges, "num_node_sample": inf["num_node_sample"],
-                   "mini_batch_size": inf["mini_batch_size"],
-                   "sampler_max_source": opts.sampler_max_source, "perplexity_interval": 50,
-                   "held_out": inf["held_out_count"], "nodes_per_step": nodes / args.steps,
-                   "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_update_phi", "avg_launch_us": 1e6 * avg_launch,
-                     "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "nodes_per_launch": nodes_per_launch, "launch_shape": smp.get_tuning()},
-        "update_phi_kernel_node_k_per_s": phi_nodes * K / phi_s if phi_s > 0 else None,
-        "timed_region_events": args.timer_mode,
-        "phase_us_per_step": {k: 1e6 * sec / bd_steps for k, (sec, n) in ph_all.items()},
-        "perplexity_last": ckpt[-1][1] if ckpt else None,
-        "stream_read_GBs": probe_bytes / probe_s / 1e9,
-        "cpu_baseline": None,
-    }
-    if rank == 0 and not args.no_cpu_baseline:
-        pi, theta, _ = smp.get_state()
-        step = smp.info["step_count"]
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cb = cpu_baseline(g, s, opts, pi, theta, step, args.cpu_budget, threads)
-        # the reference's SEQ build runs the loop on one thread
-        seq = cpu_baseline(g, s, opts, pi, theta, step, args.cpu_budget / 2, 1)
-        cb["single_thread"] = {k: seq[k] for k in ("value", "unit", "cores", "sample",
-                                                   "update_phi_node_k_per_s", "seconds")}
-        cb["host"] = host_info()
-        out["cpu_baseline"] = cb
-        del pi
-    smp.close()
-    if rank == 0:
-        print(json.dumps(out), flush=True)
-    return 0
+    return run_throughput(args)
 
 
 if __name__ == "__main__":
