@@ -218,6 +218,9 @@ def measure(smp, args, sample_nodes):
     # a contiguous read of the resident pi array: the box's streaming read
     # rate, and a read of known size for calibrating HBM counters
     probe_s, probe_bytes = smp.probe_read(5)
+    # update_phi's own row gather without its arithmetic: the practical
+    # ceiling of this access pattern on the box
+    gather_s, gather_bytes = smp.probe_gather(5)
     t_iter = (t_window - refills_in_window * t_fill) / args.steps
     sec_per_step = t_iter + t_fill / slots + t_ppx / INTERVAL
     nodes_per_step = n_window / args.steps
@@ -242,6 +245,9 @@ def measure(smp, args, sample_nodes):
                        "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS,
                        "kernel_node_k_per_s": n_window * K / phi_s if phi_s > 0 else None},
         "stream_read_GBs": probe_bytes / probe_s / 1e9,
+        "gather_probe": {"us": 1e6 * gather_s, "bytes": gather_bytes,
+                         "GBs": gather_bytes / gather_s / 1e9,
+                         "frac": gather_bytes / gather_s / 1e9 / HBM_PEAK_GBS},
         "phase_us_per_step": {k: 1e6 * sec / (2 * slots) for k, (sec, n) in phases.items()},
         "phase_ticks": {k: n for k, (sec, n) in phases.items()},
     })
@@ -428,8 +434,9 @@ def run_throughput(args, ttt_only=False):
         "components_ms": p["components_ms"],
         "window": p["window"],
         "stream_read_GBs": p["stream_read_GBs"],
+        "gather_probe": p["gather_probe"],
         other: {k: res[other][k] for k in ("value", "ms_per_step", "components_ms", "window",
-                                           "update_phi", "stream_read_GBs")},
+                                           "update_phi", "stream_read_GBs", "gather_probe")},
         "time_to_target": ttt,
         "errors_vs_fp64": errs,
         "phase_us_per_step": p["phase_us_per_step"],

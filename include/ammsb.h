@@ -195,9 +195,11 @@ int ammsb_graph_generate(const ammsb_gen_params *p, int64_t **row_ptr,
 
 /* Held-out/test split with the reference's semantics
  * (network.cc:503-614): held_out_size = ratio*E, p = held_out_size/2;
- * held-out = p random edges + p random non-edges; test = p random edges not
- * in held-out + p non-edges not in either.  Arrays allocated by the
- * library; release with ammsb_free(). */
+ * held-out = p random edges + p random non-edges; test = p + 1 random
+ * edges not in held-out (init_test_set's `if (p < 0) break` after `p--`
+ * admits one more, network.cc:566-580; p when only p remain) + p non-edges
+ * not in either.  Arrays allocated by the library; release with
+ * ammsb_free(). */
 int ammsb_split_generate(const ammsb_graph *g, double held_out_ratio,
                          uint64_t seed, int32_t **held_out_edges,
                          uint8_t **held_out_labels, int64_t *held_out_count,
@@ -350,6 +352,13 @@ int ammsb_get_tuning(ammsb_ctx *ctx, const char *name, int64_t *value);
  * of the resident pi array; *seconds = average time of one read, *bytes =
  * bytes read.  Also a read of known size for calibrating HBM counters. */
 int ammsb_probe_read(ammsb_ctx *ctx, int32_t reps, double *seconds, int64_t *bytes);
+/* Bandwidth probe (no counterpart in the reference): the random row gather
+ * of update_phi for iteration step_count -- its minibatch nodes' own rows
+ * and n+1 neighbour rows, read with update_phi's default launch shape --
+ * without the arithmetic and the row writes; `reps` launches, *seconds =
+ * average time of one, *bytes = rows and neighbour ids read.  The state is
+ * not changed; the sample-ahead ring is refilled before the next step. */
+int ammsb_probe_gather(ammsb_ctx *ctx, int32_t reps, double *seconds, int64_t *bytes);
 
 /* ---- differentiation --------------------------------------------------- */
 /* Minibatch log-likelihood of M source nodes, all pointers on the device:

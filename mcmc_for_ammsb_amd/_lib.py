@@ -181,6 +181,7 @@ def _load():
         "ammsb_loglik_bwd_noalloc": (c_i32, [c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp,
                                              c_i64, c_vp, c_vp, c_vp, c_vp]),
         "ammsb_probe_read": (c_i32, [c_vp, c_i32, P_dbl, P_i64]),
+        "ammsb_probe_gather": (c_i32, [c_vp, c_i32, P_dbl, P_i64]),
         "ammsb_set_tuning": (c_i32, [c_vp, ctypes.c_char_p, c_i64]),
         "ammsb_get_tuning": (c_i32, [c_vp, ctypes.c_char_p, P_i64]),
         "ammsb_device_ptrs": (c_i32, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp),

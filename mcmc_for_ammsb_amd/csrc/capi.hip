@@ -830,6 +830,7 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
   h0.step = 1;
   h0.ring_slots = 1;
   HIP_TRY(hipMemcpy(c->d_st, &h0, sizeof(DevState), hipMemcpyHostToDevice));
+  HIP_TRY(launch_set_state_step(c->d_st, 1, c->hp, c->stream));  // step 1 and its eps_t
 
   // ---- iteration scratch (sizes: network.cc:172-201)
   c->edge_cap = c->m + c->max_degree + 1;
@@ -1173,7 +1174,7 @@ int ammsb_set_counters(ammsb_ctx* c, int64_t step_count, int64_t average_count) 
   c->step_count = step_count;
   c->average_count = average_count;
   c->ring_valid = false;
-  HIP_TRY(launch_set_state_step(c->d_st, step_count, c->stream));
+  HIP_TRY(launch_set_state_step(c->d_st, step_count, c->hp, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return AMMSB_OK;
 }
@@ -1444,7 +1445,7 @@ int ammsb_debug_sample(ammsb_ctx* c, int64_t step, int32_t* flag, double* scale,
   if (h.error) {
     set_error("ammsb_debug_sample: device sampler error");
     // clear the flag so later calls start clean
-    (void)launch_set_state_step(c->d_st, c->step_count, c->stream);
+    (void)launch_set_state_step(c->d_st, c->step_count, c->hp, c->stream);
     (void)hipStreamSynchronize(c->stream);
     return AMMSB_ERR_INVARIANT;
   }
@@ -1588,6 +1589,45 @@ int ammsb_probe_read(ammsb_ctx* c, int32_t reps, double* seconds, int64_t* bytes
   c->event_pool.push_back(b);
   *seconds = ms * 1e-3 / reps;
   *bytes = n;
+  return AMMSB_OK;
+}
+
+int ammsb_probe_gather(ammsb_ctx* c, int32_t reps, double* seconds, int64_t* bytes) {
+  if (!c || reps < 1 || !seconds || !bytes) return AMMSB_ERR_ARG;
+  if (!c->state_ready) {
+    set_error("ammsb_probe_gather: state not initialised");
+    return AMMSB_ERR_STATE;
+  }
+  HIP_TRY(hipSetDevice(c->opt.device));
+  if (c->fill_stream) HIP_TRY(hipStreamSynchronize(c->fill_stream));
+  // the draws of iteration step_count into ring slot 0; the ring is refilled
+  // before the next iteration runs
+  c->ring_valid = false;
+  c->aring_ok = false;
+  int rc = fill_ring(c, c->step_count, 1);
+  if (rc != AMMSB_OK) return rc;
+  IterMeta mt;
+  HIP_TRY(hipMemcpyAsync(&mt, c->ring.meta, sizeof(IterMeta), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) {
+    set_error("ammsb_probe_gather: event creation failed");
+    return AMMSB_ERR_HIP;
+  }
+  const PhiArgs pa = phi_args(c);
+  HIP_TRY(launch_gather_probe(pa, c->prec, (int32_t)c->node_cap, c->d_ppx_partials, c->stream));
+  HIP_TRY(hipEventRecord(a, c->stream));
+  for (int32_t r = 0; r < reps; ++r)
+    HIP_TRY(launch_gather_probe(pa, c->prec, (int32_t)c->node_cap, c->d_ppx_partials, c->stream));
+  HIP_TRY(hipEventRecord(b, c->stream));
+  HIP_TRY(hipEventSynchronize(b));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, a, b));
+  c->event_pool.push_back(a);
+  c->event_pool.push_back(b);
+  *seconds = ms * 1e-3 / reps;
+  // own row + n+1 neighbour rows of K elements, and the n+1 neighbour ids
+  *bytes = (int64_t)mt.node_count * ((c->nnb + 1) * (int64_t)c->K * (int64_t)c->fsize + 4 * c->nnb);
   return AMMSB_OK;
 }
 
@@ -2079,7 +2119,9 @@ int ammsb_batch_create(int32_t n, const ammsb_options* opts, const ammsb_graph* 
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
     cus = 256;
   b->phi_grid = std::max(1, cus * 8);
-  b->finish_grid = std::max(1, cus * 4);
+  // six resident wavefronts per SIMD: what the FP32 finish kernel's registers
+  // allow (8 with a register bound measured the same)
+  b->finish_grid = std::max(1, cus * 6);
   *out = b;
   return AMMSB_OK;
 }

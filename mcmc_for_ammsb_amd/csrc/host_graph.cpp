@@ -329,9 +329,13 @@ extern "C" int ammsb_split_generate(const ammsb_graph* g, double ratio, uint64_t
       ho.push_back(a); ho.push_back(b); hol.push_back(0);
       ++got;
     }
-    // phase 2: test links
+    // phase 2: test links.  init_test_set (network.cc:559-590) tests
+    // `if (p < 0) break` after `p--`, so the edge that takes p from 0 to -1
+    // is still added: the test set holds p + 1 links whenever the training
+    // links allow it (its sampled batch of 2p edges nearly always does).
     j = 0;
-    for (int64_t got = 0; got < p; ++j) {
+    const int64_t test_links = E >= 2 * p + 1 ? p + 1 : p;
+    for (int64_t got = 0; got < test_links; ++j) {
       if (j > limit) { set_error("split: attempt limit (test links)"); return AMMSB_ERR_INVARIANT; }
       int32_t a, b;
       edge_at(draw_u64(key, (uint32_t)j, (uint32_t)(j >> 32), 2, TAG_SPLIT) % twoE, &a, &b);

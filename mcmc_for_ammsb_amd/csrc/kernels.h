@@ -25,6 +25,7 @@ struct DevState {
   int32_t done_count;  // k_theta workgroups finished (last one bumps step)
   int32_t ring_slots;  // slots of the sample-ahead ring
   int64_t ring_base;   // step held by slot 0 of the sample-ahead ring
+  double eps_t;        // get_eps_t of `step`, set with it (k_set_step, k_theta)
 };
 
 enum : int32_t {
@@ -239,7 +240,8 @@ hipError_t launch_init_phi(void* pi, int64_t ld, int32_t K, int32_t N, Key key,
                            int precision, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 hipError_t launch_fill_f64(double* p, double v, int64_t n, hipStream_t s);
-hipError_t launch_set_state_step(DevState* st, int64_t step, hipStream_t s);
+// DevState::step = step (with its eps_t) and the error word cleared, in stream order
+hipError_t launch_set_state_step(DevState* st, int64_t step, const DevHyper& hp, hipStream_t s);
 // DevState::ring_base and ring_slots, in stream order
 hipError_t launch_set_ring(DevState* st, int64_t base, int32_t slots, hipStream_t s);
 // Which (lanes-per-row, chunks) configuration serves K; false if K too big.
@@ -257,6 +259,10 @@ hipError_t launch_label_nbrs(DevGraph g, const int32_t* nodes, int32_t n_nodes, 
 namespace ammsb {
 // Contiguous read of `bytes` (a multiple of 16) starting at p.
 hipError_t launch_read_probe(const void* p, int64_t bytes, double* sink, hipStream_t s);
+// update_phi's row gather of the current ring iteration (default launch
+// shape) without the arithmetic or the row write.
+hipError_t launch_gather_probe(const PhiArgs& a, int precision, int32_t grid_nodes, double* sink,
+                               hipStream_t s);
 }  // namespace ammsb
 
 namespace ammsb {

@@ -786,7 +786,8 @@ __device__ __forceinline__ void phi_epilogue(const PhiArgs& a, int slot, int32_t
   const int L = LPR > 0 ? LPR : lpr_rt;
   const int K = a.K;
   const int64_t t = a.st->step;
-  const double eps_t_d = a.eps_override >= 0.0 ? a.eps_override : eps_of_step(a.hp, t);
+  // the iteration's step size, computed once per step on the device
+  const double eps_t_d = a.eps_override >= 0.0 ? a.eps_override : a.st->eps_t;
   const F eps_t = (F)eps_t_d;
   const F half_eps = eps_t / (F)2;
   const F alpha = (F)a.hp.alpha;
@@ -1093,7 +1094,10 @@ __device__ __forceinline__ void beta_body(const BetaArgs& a, const int bx) {
     }
   // Several edges per row group in flight: the edges' ids are read, then
   // all their rows are requested, then the edges are reduced.
-  constexpr int UE = CH <= 2 ? 4 : (CH <= 4 ? 2 : 1);  // fewer for big rows (registers)
+  // edges in flight per row group: fewer for big rows (registers; the
+  // launch has about one wavefront per SIMD, so occupancy is not the limit)
+  constexpr int ROWREG = EPL * (int)sizeof(F) / 4;  // VGPRs of one row per lane
+  constexpr int UE = ROWREG <= 16 ? 4 : (ROWREG < 32 ? 2 : 1);
   const int gstride = a.blocks * WAVES * RPW;
   for (int e0 = (bx * WAVES + w) * RPW + g; e0 < count; e0 += UE * gstride) {
     int32_t ea[UE], eb[UE];
@@ -1209,7 +1213,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_beta(BetaArgs a) {
 // partial rows, then the row sums meet in LDS by a pairwise tree in a fixed
 // order (deterministic).  The last workgroup to finish advances step_count,
 // so every workgroup reads the same step.
-constexpr int kThetaThreads = 1024;
+constexpr int kThetaThreads = 256;  // 1024 measured slower: fewer resident workgroups
 constexpr int kThetaK = 16;                        // k per workgroup
 constexpr int kThetaRows = kThetaThreads / kThetaK;  // partial-row strides (power of two)
 __device__ __forceinline__ void theta_body(const ThetaArgs& a, const int bx, const int gx) {
@@ -1241,7 +1245,7 @@ __device__ __forceinline__ void theta_body(const ThetaArgs& a, const int bx, con
   if (rg == 0 && k < K) {
     G0 = s_g[0][0][kl];
     G1 = s_g[0][1][kl];
-    const double eps_t = a.eps_override >= 0.0 ? a.eps_override : eps_of_step(a.hp, t);
+    const double eps_t = a.eps_override >= 0.0 ? a.eps_override : st->eps_t;
     const double scale =
         a.scale_override >= 0.0 ? a.scale_override : a.meta[ring_slot(st)].scale;
     double xi0, xi1;
@@ -1275,6 +1279,7 @@ __device__ __forceinline__ void theta_body(const ThetaArgs& a, const int bx, con
         st->nodes_total += mt.node_count;
         st->edges_total += mt.mb_count;
         st->done_count = 0;
+        st->eps_t = eps_of_step(a.hp, t + 1);
         st->step = t + 1;
       }
     }
@@ -1462,9 +1467,10 @@ __global__ void k_set_ring(DevState* st, int64_t base, int32_t slots) {
   }
 }
 
-__global__ void k_set_step(DevState* st, int64_t step) {
+__global__ void k_set_step(DevState* st, int64_t step, DevHyper hp) {
   if (threadIdx.x == 0) {
     st->step = step;
+    st->eps_t = eps_of_step(hp, step);
     st->error = 0;
   }
 }
@@ -1703,8 +1709,8 @@ hipError_t launch_set_ring(DevState* st, int64_t base, int32_t slots, hipStream_
   return hipGetLastError();
 }
 
-hipError_t launch_set_state_step(DevState* st, int64_t step, hipStream_t s) {
-  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, st, step);
+hipError_t launch_set_state_step(DevState* st, int64_t step, const DevHyper& hp, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, st, step, hp);
   return hipGetLastError();
 }
 
@@ -1755,6 +1761,82 @@ hipError_t launch_read_probe(const void* p, int64_t bytes, double* sink, hipStre
   const int64_t n = bytes / 16;
   hipLaunchKernelGGL(k_read_probe, dim3(256 * 64), dim3(256), 0, s, (const double2*)p, n, sink);
   return hipGetLastError();
+}
+
+namespace {
+// Bandwidth probe of k_update_phi's row gather: the same workgroup per
+// minibatch node, wavefront split, row groups, neighbour-id registers and
+// streaming row loads as k_update_phi (default shape), without the
+// arithmetic and without the staged row write.  The rows are summed into a
+// value stored only if it equals an impossible value, so no load can be
+// removed.
+template <typename F, int LPR, int CH, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void k_gather_probe(PhiArgs a, double* sink) {
+  constexpr int V = VT<F>::V;
+  constexpr int EPL = V * CH;
+  constexpr int RPW = 64 / LPR;
+  constexpr int STRIDE = WAVES * RPW;
+  const int slot = blockIdx.x;
+  const int rs = ring_slot(a.st);
+  const int count = a.ring.meta[rs].node_count;
+  if (slot >= count) return;
+  const int32_t* nodes = a.ring.nodes + (int64_t)rs * a.ring.node_cap;
+  const int32_t* nb_src = a.ring.nbrs + ((int64_t)rs * a.ring.node_cap + slot) * a.nnb;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int g = lane / LPR, lg = lane % LPR;
+  const F* pi = (const F*)a.pi;
+  const int K = a.K;
+  const int32_t node = nodes[slot];
+  const bool reg_ids = a.nnb <= 64 * WAVES;
+  int32_t ids = node;
+  if (reg_ids) {
+    const int r = w * RPW + lane % RPW + STRIDE * (lane / RPW);
+    if (r < a.nnb) ids = nb_src[r];
+  }
+  F row[EPL];
+  load_row<F, LPR, CH>(pi + (int64_t)node * a.ld, K, lg, row);
+  F acc = (F)0;
+#pragma unroll
+  for (int j = 0; j < EPL; ++j) acc += row[j];
+  for (int it = 0; w * RPW + it * STRIDE < a.nnb; ++it) {
+    const int b = w * RPW + g + it * STRIDE;
+    int32_t v;
+    if (reg_ids) {
+      if constexpr (RPW == 1)
+        v = __builtin_amdgcn_readlane(ids, min(it, 63));
+      else
+        v = __shfl(ids, (it * RPW + g) & 63, 64);
+    } else {
+      v = b < a.nnb ? nb_src[b] : node;
+    }
+    v = b < a.nnb ? v : node;
+    if (LPR == 64) v = __builtin_amdgcn_readfirstlane(v);
+    issue_row<F, LPR, CH, true>(pi + (int64_t)(v & 0x7fffffff) * a.ld, K, lg, row);
+#pragma unroll
+    for (int j = 0; j < EPL; ++j) acc += row[j];
+  }
+  if (acc == (F)-1.2345e30) sink[0] = (double)acc;
+}
+}  // namespace
+
+hipError_t launch_gather_probe(const PhiArgs& a, int precision, int32_t grid_nodes, double* sink,
+                               hipStream_t s) {
+  int lpr, ch;
+  if (!row_config(a.K, precision, &lpr, &ch)) return hipErrorInvalidValue;
+  if (grid_nodes <= 0) return hipSuccess;
+#define X(L, C)                                                                              \
+  if (lpr == L && ch == C) {                                                                 \
+    if (precision == 0)                                                                      \
+      hipLaunchKernelGGL((k_gather_probe<double, L, C, 4>), dim3(grid_nodes), dim3(256), 0, s, \
+                         a, sink);                                                           \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_gather_probe<float, L, C, 4>), dim3(grid_nodes), dim3(256), 0, s,  \
+                         a, sink);                                                           \
+    return hipGetLastError();                                                                \
+  }
+  AMMSB_ROW_CONFIGS(X)
+#undef X
+  return hipErrorInvalidValue;
 }
 }  // namespace ammsb
 
@@ -2714,8 +2796,10 @@ __global__ __launch_bounds__(kBatchWaves * 64) void k_update_phi_batch(
     const int npb = __builtin_amdgcn_readfirstlane(shapes[c].npb);
     const int local = it - __builtin_amdgcn_readfirstlane(s_off[c]);
     const int lpr = 4 << (cfg % 5);  // the kBatchCfgLpr / kBatchCfgCh tables
+    // neighbour rows in flight per row group: 4 of one 16-byte vector per
+    // lane, 2 of two (4 KB per wavefront either way)
     if (cfg < 5)
-      phi_batch_body<F, 1, U>(a, local, npb, lpr, s_red);
+      phi_batch_body<F, 1, 2 * U>(a, local, npb, lpr, s_red);
     else
       phi_batch_body<F, 2, U>(a, local, npb, lpr, s_red);
   }

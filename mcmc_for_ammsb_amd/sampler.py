@@ -223,6 +223,14 @@ class MCMCSamplerStochastic:
         _lib.check(_lib.lib.ammsb_probe_read(self._h, int(reps), ctypes.byref(sec), ctypes.byref(nb)))
         return sec.value, nb.value
 
+    def probe_gather(self, reps=5):
+        """(seconds, bytes) of update_phi's row gather for the current step
+        without its arithmetic: the practical ceiling of that access pattern."""
+        sec, nb = ctypes.c_double(), ctypes.c_int64()
+        _lib.check(_lib.lib.ammsb_probe_gather(self._h, int(reps), ctypes.byref(sec),
+                                               ctypes.byref(nb)))
+        return sec.value, nb.value
+
     def totals(self):
         """(nodes updated, minibatch edges) summed over all iterations."""
         n, e = ctypes.c_int64(), ctypes.c_int64()

@@ -512,6 +512,34 @@ def test_empty_inputs_and_operator_reuse():
         assert ok, (n_nodes, nnb, rel)
 
 
+def test_bandwidth_probes_leave_the_state_alone():
+    """The read probe and update_phi's gather probe (no counterpart in the
+    reference) report the bytes they read -- the gather: the step's nodes'
+    own and n+1 neighbour rows plus the ids -- change no state, and the
+    iterations after them equal those of a context that never probed."""
+    print("bandwidth probes", flush=True)
+    N, K, n = 4000, 64, 16
+    g, s, a, _ = make_pair("community", N, K, n=n, m=400, seed=9)
+    _, _, b, _ = make_pair("community", N, K, n=n, m=400, seed=9)
+    pi, theta = _state(N, K, 3)
+    a.set_state(pi, theta)
+    b.set_state(pi, theta)
+    a.step(3)
+    b.step(3)
+    t = a.info["step_count"]
+    nodes = a.debug_sample(t)["nodes"]
+    sec, nb = a.probe_gather(2)
+    assert sec > 0 and nb == len(nodes) * ((n + 2) * K * 8 + 4 * (n + 1))
+    sec, nb = a.probe_read(2)
+    assert sec > 0 and nb == N * a.info["ld"] * 8
+    for x, y in zip(a.get_state(), b.get_state()):
+        np.testing.assert_array_equal(x, y)
+    a.step(4)
+    b.step(4)
+    for x, y in zip(a.get_state(), b.get_state()):
+        np.testing.assert_array_equal(x, y)
+
+
 def test_error_paths():
     N, K = 2000, 16
     g, s = graph("community", N, K)

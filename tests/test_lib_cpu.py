@@ -194,11 +194,13 @@ def test_split_semantics():
     g = M.generate("community", N=20000, K=32, mean_degree=20.0, seed=2)
     s = M.make_split(g, 0.1, 42)
     p = int(0.1 * g.num_edges) // 2  # network.cc:72,504,562
-    assert len(s.held_out_labels) == 2 * p and len(s.test_labels) == 2 * p
-    assert s.held_out_labels.sum() == p and s.test_labels.sum() == p
+    # held-out: p links + p non-links; test: p + 1 links (init_test_set adds
+    # one more after p reaches 0, network.cc:566-580) + p non-links
+    assert len(s.held_out_labels) == 2 * p and len(s.test_labels) == 2 * p + 1
+    assert s.held_out_labels.sum() == p and s.test_labels.sum() == p + 1
     ho = {tuple(e) for e in s.held_out_edges}
     te = {tuple(e) for e in s.test_edges}
-    assert len(ho) == 2 * p and len(te) == 2 * p and not (ho & te)
+    assert len(ho) == 2 * p and len(te) == 2 * p + 1 and not (ho & te)
     for (a, b), y in zip(s.held_out_edges, s.held_out_labels):
         assert a < b and g.linked(a, b) == bool(y)
     for (a, b), y in zip(s.test_edges, s.test_labels):
