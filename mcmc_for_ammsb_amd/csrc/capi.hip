@@ -1795,7 +1795,7 @@ struct ammsb_batch {
   ThetaArgs* d_targs = nullptr;
   BatchPhiShape* d_shapes = nullptr;
   int32_t* d_beta_cfg = nullptr;
-  int32_t* d_index = nullptr;     // items, nodes, fin [n+1] each, counter
+  int32_t* d_index = nullptr;       // work layout (batch_index_ints(n) ints)
   int32_t* d_tune_index = nullptr;  // the same for one chain (autotuner)
   // Sample-ahead ring of every chain: 2R slots in two halves.  Block k =
   // iterations [ring_B + kR, ring_B + (k+1)R) lives in half k mod 2; the
@@ -1811,7 +1811,7 @@ struct ammsb_batch {
   std::vector<double> tuned_us;
   size_t mb_lds = 0, nb_lds = 0;
   int32_t max_node_cap = 0, max_beta_blocks = 0, max_K = 0;
-  int32_t phi_grid = 1, finish_grid = 1;
+  int32_t phi_grid = 1, scatter_grid = 1;
   bool use_graph = true;
   bool refill_in_order = true;  // sampler refills on the main stream (else a second stream)
   hipGraph_t graph = nullptr;  // one iteration (samplers excluded)
@@ -1820,12 +1820,18 @@ struct ammsb_batch {
 
 namespace {
 
+// Work layout in one device buffer of int32: order [n], items [n+1],
+// pub [n+1], counter, then info [n] (int4, 16-byte aligned).
+size_t batch_index_info_offset(int32_t n) { return ((size_t)3 * n + 3 + 3) / 4 * 4; }
+size_t batch_index_ints(int32_t n) { return batch_index_info_offset(n) + (size_t)4 * n; }
+
 BatchIndex batch_index(int32_t* base, int32_t n) {
   BatchIndex ix;
-  ix.items = base;
-  ix.nodes = base + (n + 1);
-  ix.fin = base + 2 * (n + 1);
-  ix.counter = base + 3 * (n + 1);
+  ix.order = base;
+  ix.items = base + n;
+  ix.pub = base + 2 * n + 1;
+  ix.counter = base + 3 * n + 2;
+  ix.info = reinterpret_cast<int4*>(base + batch_index_info_offset(n));
   return ix;
 }
 
@@ -1856,8 +1862,29 @@ void free_batch(ammsb_batch* b) {
   delete b;
 }
 
+// Work order of the batched update_phi: chains whose work items move the
+// most row bytes first (npb nodes x nnb rows of K x sizeof(Float), in
+// 64-byte lines), so that the last items handed out are the short ones and
+// the persistent workgroups finish together.  Ties keep the chain order.
+std::vector<int32_t> batch_work_order(const ammsb_batch* b) {
+  std::vector<int64_t> cost(b->n);
+  for (int32_t i = 0; i < b->n; ++i) {
+    const ammsb_ctx* c = b->chains[i];
+    const int64_t row = ((int64_t)c->K * (c->prec == 0 ? 8 : 4) + 63) / 64;
+    cost[i] = (int64_t)b->shapes[i].npb * c->nnb * row;
+  }
+  std::vector<int32_t> order(b->n);
+  for (int32_t i = 0; i < b->n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
+  return order;
+}
+
 int batch_upload_shapes(ammsb_batch* b) {
+  const std::vector<int32_t> order = batch_work_order(b);
   HIP_TRY(hipMemcpyAsync(b->d_shapes, b->shapes.data(), sizeof(BatchPhiShape) * b->n,
+                         hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipMemcpyAsync(b->d_index, order.data(), sizeof(int32_t) * b->n,
                          hipMemcpyHostToDevice, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   return AMMSB_OK;
@@ -1915,8 +1942,8 @@ int batch_enqueue_iteration(ammsb_batch* b) {
   HIP_TRY(launch_batch_index(b->d_pargs, b->d_shapes, b->d_beta_cfg, b->n, ix, b->stream));
   HIP_TRY(launch_update_phi_batch(b->d_pargs, b->d_shapes, b->n, ix, b->prec, b->phi_grid,
                                   b->stream));
-  HIP_TRY(launch_phi_finish_batch(b->d_pargs, b->d_beta_cfg, b->n, ix, b->prec, b->finish_grid,
-                                  b->stream));
+  HIP_TRY(launch_scatter_batch(b->d_pargs, b->d_beta_cfg, b->n, ix, b->prec, b->scatter_grid,
+                               b->stream));
   HIP_TRY(launch_beta_batch(b->d_bargs, b->d_beta_cfg, b->n, b->max_beta_blocks, b->prec,
                             b->stream));
   HIP_TRY(launch_theta_batch(b->d_targs, b->n, b->max_K, b->stream));
@@ -2096,9 +2123,10 @@ int ammsb_batch_create(int32_t n, const ammsb_options* opts, const ammsb_graph* 
   if ((rc = dalloc(&b->d_sargs, n)) || (rc = dalloc(&b->d_nargs, n)) ||
       (rc = dalloc(&b->d_pargs, n)) || (rc = dalloc(&b->d_bargs, n)) ||
       (rc = dalloc(&b->d_targs, n)) || (rc = dalloc(&b->d_shapes, n)) ||
-      (rc = dalloc(&b->d_beta_cfg, n)) || (rc = dalloc(&b->d_index, 3 * (n + 1) + 1)) ||
-      (rc = dalloc(&b->d_tune_index, 3 * 2 + 1)))
+      (rc = dalloc(&b->d_beta_cfg, n)) || (rc = dalloc(&b->d_index, batch_index_ints(n))) ||
+      (rc = dalloc(&b->d_tune_index, batch_index_ints(1))))
     return fail(rc);
+  static const int32_t kZero = 0;
   const struct {
     void* d;
     const void* h;
@@ -2108,7 +2136,8 @@ int ammsb_batch_create(int32_t n, const ammsb_options* opts, const ammsb_graph* 
             {b->d_pargs, pargs.data(), sizeof(PhiArgs) * n},
             {b->d_bargs, bargs.data(), sizeof(BetaArgs) * n},
             {b->d_targs, targs.data(), sizeof(ThetaArgs) * n},
-            {b->d_beta_cfg, beta_cfg.data(), sizeof(int32_t) * n}};
+            {b->d_beta_cfg, beta_cfg.data(), sizeof(int32_t) * n},
+            {b->d_tune_index, &kZero, sizeof(int32_t)}};  // the tuner's order: chain 0
   for (const auto& u : up)
     if (hipMemcpy(u.d, u.h, u.bytes, hipMemcpyHostToDevice) != hipSuccess) {
       set_error("ammsb_batch_create: upload of kernel arguments failed");
@@ -2118,10 +2147,10 @@ int ammsb_batch_create(int32_t n, const ammsb_options* opts, const ammsb_graph* 
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
     cus = 256;
-  b->phi_grid = std::max(1, cus * 8);
-  // six resident wavefronts per SIMD: what the FP32 finish kernel's registers
-  // allow (8 with a register bound measured the same)
-  b->finish_grid = std::max(1, cus * 6);
+  // persistent: as many workgroups as stay resident
+  b->phi_grid = std::max(1, cus * update_phi_batch_blocks_per_cu(n, b->prec));
+  // eight resident wavefronts per SIMD for the copy
+  b->scatter_grid = std::max(1, cus * 8);
   *out = b;
   return AMMSB_OK;
 }

@@ -333,13 +333,18 @@ struct BatchPhiShape {
   int32_t cfg;
   int32_t npb;
 };
-// Per-iteration offsets: items[c] = first work item of chain c (items[n] =
-// total), nodes[c] likewise for minibatch nodes, fin[c] for the finish
-// kernel's wavefront tasks; counter = work queue head.
+// Per-iteration work layout of a batch.  Chains are laid out in a work
+// order set by the host (order[p] = the chain at position p, the most
+// expensive work items first, so that the last items to finish are short):
+// items[p] = first work item of position p (items[n] = total), pub[p]
+// likewise for the batched scatter's wavefront tasks, info[p] = (chain,
+// ring slot, minibatch node count, 0) of the current iteration; counter =
+// work queue head.
 struct BatchIndex {
+  const int32_t* order;
   int32_t* items;
-  int32_t* nodes;
-  int32_t* fin;
+  int32_t* pub;
+  int4* info;
   int32_t* counter;
 };
 // Samplers of `fills` iterations ahead into ring slots 0..fills-1.
@@ -349,16 +354,18 @@ hipError_t launch_minibatch_batch(const SamplerArgs* args, int n, int64_t t0, in
                                   int slots, size_t lds, hipStream_t s);
 hipError_t launch_neighbors_batch(const NeighborArgs* args, int n, int slot0, int fills, int slots,
                                   int max_node_cap, size_t lds, hipStream_t s);
-// cfg: each chain's update_beta row shape, also the finish kernel's
+// cfg: each chain's update_beta row shape, also the batched scatter's
 hipError_t launch_batch_index(const PhiArgs* args, const BatchPhiShape* shapes,
                               const int32_t* cfg, int n, BatchIndex ix, hipStream_t s);
 hipError_t launch_update_phi_batch(const PhiArgs* args, const BatchPhiShape* shapes, int n,
                                    BatchIndex ix, int precision, int grid, hipStream_t s);
-// update_phi of the batch in two launches: the persistent gather writes each
-// node's gradient sums to its staging row, the finish applies the phi step
-// and writes pi in place
-hipError_t launch_phi_finish_batch(const PhiArgs* args, const int32_t* cfg, int n,
-                                   BatchIndex ix, int precision, int grid, hipStream_t s);
+// Resident workgroups per CU of the batched update_phi for n chains (its
+// persistent grid is this times the CU count).
+int update_phi_batch_blocks_per_cu(int n, int precision);
+// update_pi of the batch: the persistent gather stages each node's new row
+// (phi step applied, phi_sum included), the scatter writes it into pi
+hipError_t launch_scatter_batch(const PhiArgs* args, const int32_t* cfg, int n, BatchIndex ix,
+                                int precision, int grid, hipStream_t s);
 hipError_t launch_beta_batch(const BetaArgs* args, const int32_t* cfg, int n, int max_blocks,
                              int precision, hipStream_t s);
 hipError_t launch_theta_batch(const ThetaArgs* args, int n, int max_K, hipStream_t s);

@@ -66,6 +66,48 @@ __device__ __forceinline__ void load_vec<float>(const float* p, float* out) {
   out[3] = v.w;
 }
 
+template <typename F>
+__device__ __forceinline__ void store_vec(F* p, const F* in);
+template <>
+__device__ __forceinline__ void store_vec<double>(double* p, const double* in) {
+  *reinterpret_cast<double2*>(p) = make_double2(in[0], in[1]);
+}
+template <>
+__device__ __forceinline__ void store_vec<float>(float* p, const float* in) {
+  *reinterpret_cast<float4*>(p) = make_float4(in[0], in[1], in[2], in[3]);
+}
+typedef double st_d2 __attribute__((ext_vector_type(2)));
+typedef float st_f4 __attribute__((ext_vector_type(4)));
+// 16-byte vector types the nontemporal builtins accept
+template <typename F>
+struct NTV;
+template <>
+struct NTV<double> {
+  typedef st_d2 type;
+};
+template <>
+struct NTV<float> {
+  typedef st_f4 type;
+};
+template <typename F>
+__device__ __forceinline__ void store_vec_nt(F* p, const F* in);
+template <>
+__device__ __forceinline__ void store_vec_nt<double>(double* p, const double* in) {
+  st_d2 v;
+  v.x = in[0];
+  v.y = in[1];
+  __builtin_nontemporal_store(v, reinterpret_cast<st_d2*>(p));
+}
+template <>
+__device__ __forceinline__ void store_vec_nt<float>(float* p, const float* in) {
+  st_f4 v;
+  v.x = in[0];
+  v.y = in[1];
+  v.z = in[2];
+  v.w = in[3];
+  __builtin_nontemporal_store(v, reinterpret_cast<st_f4*>(p));
+}
+
 // Streaming (non-temporal) 16-byte loads, for rows read once per use.
 typedef double nt_d2 __attribute__((ext_vector_type(2)));
 typedef float nt_f4 __attribute__((ext_vector_type(4)));
@@ -776,18 +818,17 @@ __global__ __launch_bounds__(WAVES * 64) void k_neighbors(NeighborArgs a, int32_
 // phi step of one node (mcmc_sampler_stochastic_distr.cc:1207-1226) from the
 // row group's accumulated gradient sums acc (x_bk/Z_b - 1 summed over the
 // neighbours), written with its renormalised pi and phi_sum to stg (K+1).
-// LPR = 0: the row group has lpr_rt lanes (a uniform runtime value).
+// LPR = 0: the row group has lpr_rt lanes (a uniform runtime value).  t is
+// the step (the noise counter) and eps_t_d its step size (phi_step_size),
+// read by the caller before its row loads so that their latency is hidden.
 template <typename F, int LPR, int CH>
 __device__ __forceinline__ void phi_epilogue(const PhiArgs& a, int slot, int32_t node,
                                              const F phi_sum, const F* pii, const F* acc, int lg,
-                                             F* stg, int lpr_rt = 0) {
+                                             F* stg, int64_t t, double eps_t_d, int lpr_rt = 0) {
   constexpr int V = VT<F>::V;
   constexpr int EPL = V * CH;
   const int L = LPR > 0 ? LPR : lpr_rt;
   const int K = a.K;
-  const int64_t t = a.st->step;
-  // the iteration's step size, computed once per step on the device
-  const double eps_t_d = a.eps_override >= 0.0 ? a.eps_override : a.st->eps_t;
   const F eps_t = (F)eps_t_d;
   const F half_eps = eps_t / (F)2;
   const F alpha = (F)a.hp.alpha;
@@ -829,14 +870,31 @@ __device__ __forceinline__ void phi_epilogue(const PhiArgs& a, int slot, int32_t
   else
     s = group_sum_rt(s, lpr_rt);
   const double inv_s = 1.0 / s;
+  // one non-temporal 16-byte store per vector (the row is 16-byte aligned;
+  // streaming stores cost the concurrent row gathers of other workgroups
+  // less: C3 FP32 update_phi 383 -> 373 us); a vector that reaches past the
+  // row is stored element by element, so that phi_sum at K is written once
 #pragma unroll
-  for (int c = 0; c < CH; ++c)
+  for (int c = 0; c < CH; ++c) {
+    const int k0 = (c * L + lg) * V;
+    F out[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const int k = (c * L + lg) * V + v;
-      if (k < K) stg[k] = (F)((double)phin[c * V + v] * inv_s);
+    for (int v = 0; v < V; ++v) out[v] = (F)((double)phin[c * V + v] * inv_s);
+    if (k0 + V <= K) {
+      store_vec_nt<F>(stg + k0, out);
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        if (k0 + v < K) stg[k0 + v] = out[v];
     }
+  }
   if (lg == 0) stg[K] = (F)s;
+}
+
+// The iteration's step size: the injected one, else the one computed once
+// per step on the device.
+__device__ __forceinline__ double phi_step_size(const PhiArgs& a) {
+  return a.eps_override >= 0.0 ? a.eps_override : a.st->eps_t;
 }
 
 // EPI: the phi step is applied here and the new row staged (k_scatter
@@ -884,6 +942,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
     if (r < a.nnb) ids = nb_src[r];
   }
   const F phi_sum = prow[K];
+  const int64_t t = EPI ? a.st->step : 0;
+  const double eps_t = EPI ? phi_step_size(a) : 0.0;
 
   F pii[EPL], f1[EPL], acc[EPL];
   load_row<F, LPR, CH>(prow, K, lg, pii);
@@ -979,7 +1039,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
 
   if constexpr (EPI) {
     phi_epilogue<F, LPR, CH>(a, slot, node, phi_sum, pii, acc, lg,
-                             (F*)a.staging + (int64_t)slot * a.ld);
+                             (F*)a.staging + (int64_t)slot * a.ld, t, eps_t);
   } else {
     F* stg = (F*)a.staging + (int64_t)slot * a.ld;
 #pragma unroll
@@ -1019,7 +1079,8 @@ __global__ __launch_bounds__(256) void k_phi_finish(PhiArgs a) {
   F pii[EPL], acc[EPL];
   load_row<F, LPR, CH>(prow, a.K, lg, pii);
   load_row<F, LPR, CH>((const F*)a.staging + (int64_t)slot * a.ld, a.K, lg, acc);
-  phi_epilogue<F, LPR, CH>(a, slot, node, prow[a.K], pii, acc, lg, prow);
+  phi_epilogue<F, LPR, CH>(a, slot, node, prow[a.K], pii, acc, lg, prow, a.st->step,
+                           phi_step_size(a));
 }
 
 // Publish staged rows: pi[nodes[slot]] = staging[slot] (K+1 elements).
@@ -1040,10 +1101,12 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   if (slot >= count) return;
   constexpr int V = VT<F>::V;
   const int nv = (a.K + 1 + V - 1) / V;  // row and phi_sum; ld is a multiple of 128 B
-  typedef typename VT<F>::type T;
+  typedef typename NTV<F>::type T;
   const T* src = reinterpret_cast<const T*>((const F*)a.staging + (int64_t)slot * a.ld);
   T* dst = reinterpret_cast<T*>((F*)a.pi + (int64_t)nodes[slot] * a.ld);
-  for (int i = lane; i < nv; i += 64) dst[i] = src[i];
+  // non-temporal stores: the rows are read again only by the next iteration
+  // (C3 FP32: the next update_phi 373 -> 368 us against plain stores)
+  for (int i = lane; i < nv; i += 64) __builtin_nontemporal_store(src[i], dst + i);
 }
 
 // ---------------------------------------------------------------------------
@@ -2645,11 +2708,13 @@ __device__ __forceinline__ int chain_of(const int32_t* off, int n, int item) {
 // 64/lpr spreads a node over several wavefronts.  Each group streams whole
 // neighbour rows (U in flight), reduces Z_b over its lanes and accumulates
 // x_bk/Z_b - 1; the node's groups are combined by a butterfly inside the
-// wavefront, then across its wavefronts through LDS.  The sums go to the
-// node's staging row; k_phi_finish_batch applies the phi step.
+// wavefront, then across its wavefronts through LDS.  The node's leader
+// group applies the phi step (noise, renormalisation) and stages the new
+// row with its phi_sum; k_scatter_batch publishes it once every gather of
+// the iteration is done.
 template <typename F, int CH, int U>
 __device__ __forceinline__ void phi_batch_body(const PhiArgs& a, int item, int npb, int lpr,
-                                               F* s_red) {
+                                               int rs, int count, F* s_red) {
   constexpr int V = VT<F>::V;
   constexpr int EPL = V * CH;
   const int rpw = 64 / lpr;
@@ -2659,8 +2724,6 @@ __device__ __forceinline__ void phi_batch_body(const PhiArgs& a, int item, int n
   const int gpn = G / npb;
   const int gid = w * rpw + g;
   const int j = gid / gpn, r = gid % gpn;
-  const int rs = ring_slot(a.st);
-  const int count = a.ring.meta[rs].node_count;
   const int first = item * npb;  // < count: the item exists
   const int slot = first + j;
   const bool active = slot < count;
@@ -2680,6 +2743,9 @@ __device__ __forceinline__ void phi_batch_body(const PhiArgs& a, int item, int n
   const F* prow = pi + (int64_t)node * a.ld;
 #pragma unroll
   for (int c = 0; c < CH; ++c) load_vec<F>(prow + (koff[c] < K ? koff[c] : 0), pii + c * V);
+  const F phi_sum = prow[K];
+  const int64_t t = a.st->step;
+  const double eps_t = phi_step_size(a);
 #pragma unroll
   for (int c = 0; c < CH; ++c)
 #pragma unroll
@@ -2764,138 +2830,130 @@ __device__ __forceinline__ void phi_batch_body(const PhiArgs& a, int item, int n
     leader = (w % wpn) == 0 && g == 0;
   }
   if (!leader || !active) return;
-  F* stg = (F*)a.staging + (int64_t)slot * a.ld;
-#pragma unroll
-  for (int c = 0; c < CH; ++c)
-#pragma unroll
-    for (int v = 0; v < V; ++v)
-      if (koff[c] + v < K) stg[koff[c] + v] = acc[c * V + v];
+  phi_epilogue<F, 0, CH>(a, slot, node, phi_sum, pii, acc, lg,
+                         (F*)a.staging + (int64_t)slot * a.ld, t, eps_t, lpr);
 }
 
+// The work layout of the n positions staged in dynamic LDS: info [n] (int4)
+// then offsets [n+1] (int32).
+__host__ __device__ constexpr size_t batch_layout_lds(int n) {
+  return (size_t)n * sizeof(int4) + (size_t)(n + 1) * sizeof(int32_t);
+}
+
+// Resident wavefronts per SIMD: five in FP32 (registers bounded to 94; six
+// would spill), three in FP64 (152 registers; five would spill).
 template <typename F, int U>
-__global__ __launch_bounds__(kBatchWaves * 64) void k_update_phi_batch(
+__global__ __launch_bounds__(kBatchWaves * 64)
+__attribute__((amdgpu_waves_per_eu(sizeof(F) == 4 ? 5 : 3))) void k_update_phi_batch(
     const PhiArgs* __restrict__ args, const BatchPhiShape* __restrict__ shapes, int n,
     BatchIndex ix) {
   __shared__ F s_red[kBatchWaves * 8 * 64];
-  __shared__ int32_t s_off[kMaxBatchChains + 1];
   __shared__ int s_item;
+  extern __shared__ int4 s_info[];  // batch_layout_lds(n) bytes
+  int32_t* s_off = reinterpret_cast<int32_t*>(s_info + n);
   for (int i = threadIdx.x; i <= n; i += blockDim.x) s_off[i] = ix.items[i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s_info[i] = ix.info[i];
+  int next = 0;
+  if (threadIdx.x == 0) next = atomicAdd(ix.counter, 1);
   __syncthreads();
   const int total = s_off[n];
   while (true) {
-    if (threadIdx.x == 0) s_item = atomicAdd(ix.counter, 1);
+    if (threadIdx.x == 0) s_item = next;
     __syncthreads();
     // uniform values in scalar registers: the chain's arguments are then
     // scalar loads and the shape switch a scalar branch
     const int it = __builtin_amdgcn_readfirstlane(s_item);
     __syncthreads();
     if (it >= total) break;
-    const int c = __builtin_amdgcn_readfirstlane(chain_of(s_off, n, it));
+    // claim the next item now: its index arrives while this one runs
+    if (threadIdx.x == 0) next = atomicAdd(ix.counter, 1);
+    const int p = __builtin_amdgcn_readfirstlane(chain_of(s_off, n, it));
+    const int c = __builtin_amdgcn_readfirstlane(s_info[p].x);
+    const int rs = __builtin_amdgcn_readfirstlane(s_info[p].y);
+    const int count = __builtin_amdgcn_readfirstlane(s_info[p].z);
     const PhiArgs& a = args[c];
     const int cfg = __builtin_amdgcn_readfirstlane(shapes[c].cfg);
     const int npb = __builtin_amdgcn_readfirstlane(shapes[c].npb);
-    const int local = it - __builtin_amdgcn_readfirstlane(s_off[c]);
+    const int local = it - __builtin_amdgcn_readfirstlane(s_off[p]);
     const int lpr = 4 << (cfg % 5);  // the kBatchCfgLpr / kBatchCfgCh tables
     // neighbour rows in flight per row group: 4 of one 16-byte vector per
     // lane, 2 of two (4 KB per wavefront either way)
     if (cfg < 5)
-      phi_batch_body<F, 1, 2 * U>(a, local, npb, lpr, s_red);
+      phi_batch_body<F, 1, 2 * U>(a, local, npb, lpr, rs, count, s_red);
     else
-      phi_batch_body<F, 2, U>(a, local, npb, lpr, s_red);
+      phi_batch_body<F, 2, U>(a, local, npb, lpr, rs, count, s_red);
   }
 }
 
-// Work-item and node offsets of the current iteration (one workgroup), and
-// the work queue reset: runs after the samplers, before update_phi.
+// Work layout of the current iteration (one workgroup): per position of
+// the work order, the chain's ring slot and node count, its first work item
+// and scatter task, and the work queue reset.  Runs after the samplers,
+// before update_phi.
 __global__ __launch_bounds__(1024) void k_batch_index(const PhiArgs* __restrict__ args,
                                                       const BatchPhiShape* __restrict__ shapes,
                                                       const int32_t* __restrict__ cfg, int n,
                                                       BatchIndex ix) {
-  __shared__ int32_t s_items[kMaxBatchChains], s_nodes[kMaxBatchChains],
-      s_fin[kMaxBatchChains];
-  const int c = threadIdx.x;
-  if (c < n) {
+  __shared__ int32_t s_items[kMaxBatchChains], s_pub[kMaxBatchChains];
+  const int p = threadIdx.x;
+  if (p < n) {
+    const int c = ix.order[p];
     const PhiArgs& a = args[c];
-    const int cnt = a.ring.meta[ring_slot(a.st)].node_count;
-    const int rpw = 64 / (4 << (cfg[c] % 5));  // finish: row groups per wavefront
-    s_nodes[c] = cnt;
-    s_items[c] = (cnt + shapes[c].npb - 1) / shapes[c].npb;
-    s_fin[c] = (cnt + rpw - 1) / rpw;
+    const int rs = ring_slot(a.st);
+    const int cnt = a.ring.meta[rs].node_count;
+    const int rpw = 64 / (4 << (cfg[c] % 5));  // scatter: row groups per wavefront
+    s_items[p] = (cnt + shapes[c].npb - 1) / shapes[c].npb;
+    s_pub[p] = (cnt + rpw - 1) / rpw;
+    ix.info[p] = make_int4(c, rs, cnt, 0);
   }
   __syncthreads();
-  if (c == 0) {
-    int32_t ti = 0, tn = 0, tf = 0;
+  if (p == 0) {
+    int32_t ti = 0, tp = 0;
     for (int i = 0; i < n; ++i) {
       ix.items[i] = ti;
-      ix.nodes[i] = tn;
-      ix.fin[i] = tf;
+      ix.pub[i] = tp;
       ti += s_items[i];
-      tn += s_nodes[i];
-      tf += s_fin[i];
+      tp += s_pub[i];
     }
     ix.items[n] = ti;
-    ix.nodes[n] = tn;
-    ix.fin[n] = tf;
+    ix.pub[n] = tp;
     *ix.counter = 0;
   }
 }
 
-// phi step + update_pi of every chain's minibatch nodes: a wavefront takes
-// 64/lpr consecutive nodes of one chain, one per row group of lpr lanes
-// (the chain's update_beta row shape).  Reads the node's row and the
-// gradient sums staged by k_update_phi_batch and writes the new row into
-// pi.  All neighbour-row reads of the iteration are complete (previous
-// launch), as the reference refreshes pi only after the node loop
-// (mcmc_sampler_stochastic.cc:162-166).
-template <typename F, int CH>
-__device__ __forceinline__ void phi_finish_body(const PhiArgs& a, int slot, int lg, int lpr) {
-  constexpr int V = VT<F>::V;
-  constexpr int EPL = V * CH;
-  const int rs = ring_slot(a.st);
-  const int32_t node = a.ring.nodes[(int64_t)rs * a.ring.node_cap + slot];
-  F* prow = (F*)a.pi + (int64_t)node * a.ld;
-  const F* acc_row = (const F*)a.staging + (int64_t)slot * a.ld;
-  const int K = a.K;
-  F pii[EPL], acc[EPL];
-#pragma unroll
-  for (int c = 0; c < CH; ++c) {
-    const int k0 = (c * lpr + lg) * V;
-    load_vec<F>(prow + (k0 < K ? k0 : 0), pii + c * V);
-    load_vec<F>(acc_row + (k0 < K ? k0 : 0), acc + c * V);
-  }
-#pragma unroll
-  for (int c = 0; c < CH; ++c)
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const bool in = (c * lpr + lg) * V + v < K;
-      pii[c * V + v] = in ? pii[c * V + v] : (F)0;
-      acc[c * V + v] = in ? acc[c * V + v] : (F)0;
-    }
-  phi_epilogue<F, 0, CH>(a, slot, node, prow[a.K], pii, acc, lg, prow, lpr);
-}
-
+// update_pi of every chain's minibatch nodes: pi[node] = staged row (K
+// elements and phi_sum, in 16-byte vectors; ld is a multiple of 128 B).  A
+// wavefront takes 64/lpr consecutive nodes of one chain, one per row group
+// of lpr lanes (the chain's update_beta row shape).  Runs after every
+// neighbour-row read of the iteration (previous launch), as the reference
+// refreshes pi only after the node loop (mcmc_sampler_stochastic.cc:162-166).
 template <typename F>
-__global__ __launch_bounds__(256) void k_phi_finish_batch(const PhiArgs* __restrict__ args,
-                                                          const int32_t* __restrict__ cfg, int n,
-                                                          BatchIndex ix) {
-  __shared__ int32_t s_off[kMaxBatchChains + 1];
-  for (int i = threadIdx.x; i <= n; i += blockDim.x) s_off[i] = ix.fin[i];
+__global__ __launch_bounds__(256) void k_scatter_batch(const PhiArgs* __restrict__ args,
+                                                       const int32_t* __restrict__ cfg, int n,
+                                                       BatchIndex ix) {
+  constexpr int V = VT<F>::V;
+  typedef typename NTV<F>::type T;
+  extern __shared__ int4 s_info[];  // batch_layout_lds(n) bytes
+  int32_t* s_off = reinterpret_cast<int32_t*>(s_info + n);
+  for (int i = threadIdx.x; i <= n; i += blockDim.x) s_off[i] = ix.pub[i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s_info[i] = ix.info[i];
   __syncthreads();
   const int total = s_off[n];
   const int lane = threadIdx.x & 63;
   const int gw = blockIdx.x * 4 + (threadIdx.x >> 6), nw = gridDim.x * 4;
   for (int it = gw; it < total; it += nw) {
-    const int c = __builtin_amdgcn_readfirstlane(chain_of(s_off, n, it));
-    const int cf = __builtin_amdgcn_readfirstlane(cfg[c]);
-    const int lpr = 4 << (cf % 5);  // the kBatchCfgLpr / kBatchCfgCh tables
+    const int p = __builtin_amdgcn_readfirstlane(chain_of(s_off, n, it));
+    const int c = __builtin_amdgcn_readfirstlane(s_info[p].x);
+    const int rs = __builtin_amdgcn_readfirstlane(s_info[p].y);
+    const int count = __builtin_amdgcn_readfirstlane(s_info[p].z);
+    const int lpr = 4 << (__builtin_amdgcn_readfirstlane(cfg[c]) % 5);  // kBatchCfgLpr
     const PhiArgs& a = args[c];
-    const int count = a.ring.meta[ring_slot(a.st)].node_count;
-    const int slot = (it - __builtin_amdgcn_readfirstlane(s_off[c])) * (64 / lpr) + lane / lpr;
+    const int slot = (it - __builtin_amdgcn_readfirstlane(s_off[p])) * (64 / lpr) + lane / lpr;
     if (slot >= count) continue;
-    if (cf < 5)
-      phi_finish_body<F, 1>(a, slot, lane % lpr, lpr);
-    else
-      phi_finish_body<F, 2>(a, slot, lane % lpr, lpr);
+    const int32_t node = a.ring.nodes[(int64_t)rs * a.ring.node_cap + slot];
+    const int nv = (a.K + 1 + V - 1) / V;
+    const T* src = reinterpret_cast<const T*>((const F*)a.staging + (int64_t)slot * a.ld);
+    T* dst = reinterpret_cast<T*>((F*)a.pi + (int64_t)node * a.ld);
+    for (int i = lane % lpr; i < nv; i += lpr) __builtin_nontemporal_store(src[i], dst + i);
   }
 }
 
@@ -2979,22 +3037,36 @@ hipError_t launch_batch_index(const PhiArgs* args, const BatchPhiShape* shapes,
 hipError_t launch_update_phi_batch(const PhiArgs* args, const BatchPhiShape* shapes, int n,
                                    BatchIndex ix, int precision, int grid, hipStream_t s) {
   if (n < 1 || n > kMaxBatchChains || grid < 1) return hipErrorInvalidValue;
+  const size_t lds = batch_layout_lds(n);
   if (precision == 0)
-    hipLaunchKernelGGL((k_update_phi_batch<double, 2>), dim3(grid), dim3(kBatchWaves * 64), 0, s,
-                       args, shapes, n, ix);
+    hipLaunchKernelGGL((k_update_phi_batch<double, 2>), dim3(grid), dim3(kBatchWaves * 64), lds,
+                       s, args, shapes, n, ix);
   else
-    hipLaunchKernelGGL((k_update_phi_batch<float, 2>), dim3(grid), dim3(kBatchWaves * 64), 0, s,
+    hipLaunchKernelGGL((k_update_phi_batch<float, 2>), dim3(grid), dim3(kBatchWaves * 64), lds, s,
                        args, shapes, n, ix);
   return hipGetLastError();
 }
 
-hipError_t launch_phi_finish_batch(const PhiArgs* args, const int32_t* cfg, int n,
-                                   BatchIndex ix, int precision, int grid, hipStream_t s) {
+int update_phi_batch_blocks_per_cu(int n, int precision) {
+  int nb = 0;
+  const size_t lds = batch_layout_lds(n);
+  const hipError_t e =
+      precision == 0
+          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb, (const void*)k_update_phi_batch<double, 2>, kBatchWaves * 64, lds)
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb, (const void*)k_update_phi_batch<float, 2>, kBatchWaves * 64, lds);
+  return e == hipSuccess && nb > 0 ? nb : 1;
+}
+
+hipError_t launch_scatter_batch(const PhiArgs* args, const int32_t* cfg, int n, BatchIndex ix,
+                                int precision, int grid, hipStream_t s) {
   if (n < 1 || n > kMaxBatchChains || grid < 1) return hipErrorInvalidValue;
+  const size_t lds = batch_layout_lds(n);
   if (precision == 0)
-    hipLaunchKernelGGL(k_phi_finish_batch<double>, dim3(grid), dim3(256), 0, s, args, cfg, n, ix);
+    hipLaunchKernelGGL(k_scatter_batch<double>, dim3(grid), dim3(256), lds, s, args, cfg, n, ix);
   else
-    hipLaunchKernelGGL(k_phi_finish_batch<float>, dim3(grid), dim3(256), 0, s, args, cfg, n, ix);
+    hipLaunchKernelGGL(k_scatter_batch<float>, dim3(grid), dim3(256), lds, s, args, cfg, n, ix);
   return hipGetLastError();
 }
 
