@@ -367,6 +367,7 @@ PhiArgs phi_args(const ammsb_ctx* c) {
   pa.noise = nullptr;
   pa.key = c->key;
   pa.staging = c->d_staging;
+  pa.ring_slot_host = -1;
   return pa;
 }
 
@@ -381,6 +382,7 @@ ScatterArgs scatter_args(const ammsb_ctx* c) {
   sc.nodes = nullptr;
   sc.count_host = 0;
   sc.staging = c->d_staging;
+  sc.ring_slot_host = -1;
   return sc;
 }
 
@@ -400,6 +402,7 @@ BetaArgs beta_args(const ammsb_ctx* c) {
   ba.epsilon = c->hp.epsilon;
   ba.partials = c->d_beta_partials;
   ba.blocks = c->beta_blocks;
+  ba.ring_slot_host = -1;
   return ba;
 }
 
@@ -507,21 +510,30 @@ int enqueue_iteration(ammsb_ctx* c) {
   }
   const bool split = c->phi_split && c->phi_waves == kPhiWavesDefault &&
                      c->phi_unroll == kPhiUnrollDefault;
+  // the ring holds [ring_base, ring_base + slots) here (the sampler set the
+  // device's ring_base to the same step), so the slot is known on the host
+  const int32_t rs = c->async_ring ? -1 : (int32_t)(c->step_count - c->ring_base);
+  PhiArgs pa = phi_args(c);
+  pa.ring_slot_host = rs;
+  ScatterArgs sa = scatter_args(c);
+  sa.ring_slot_host = rs;
+  BetaArgs ba = beta_args(c);
+  ba.ring_slot_host = rs;
   {
     PhaseScope ps(c, PH_UPDATE_PHI);
-    HIP_TRY(launch_update_phi(phi_args(c), c->prec, (int32_t)c->node_cap, c->stream, c->phi_waves,
+    HIP_TRY(launch_update_phi(pa, c->prec, (int32_t)c->node_cap, c->stream, c->phi_waves,
                               c->phi_unroll, split));
   }
   {
     PhaseScope ps(c, PH_UPDATE_PI);
     if (split)
-      HIP_TRY(launch_phi_finish(phi_args(c), c->prec, (int32_t)c->node_cap, c->stream));
+      HIP_TRY(launch_phi_finish(pa, c->prec, (int32_t)c->node_cap, c->stream));
     else
-      HIP_TRY(launch_scatter(scatter_args(c), c->prec, (int32_t)c->node_cap, c->stream));
+      HIP_TRY(launch_scatter(sa, c->prec, (int32_t)c->node_cap, c->stream));
   }
   {
     PhaseScope ps(c, PH_UPDATE_BETA);
-    HIP_TRY(launch_beta(beta_args(c), c->prec, c->stream));
+    HIP_TRY(launch_beta(ba, c->prec, c->stream));
     HIP_TRY(launch_theta(theta_args(c), c->stream));
   }
   if (c->async_ring) return aring_after(c);

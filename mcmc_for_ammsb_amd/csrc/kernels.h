@@ -92,6 +92,12 @@ struct Ring {
 __device__ __forceinline__ int ring_slot(const DevState* st) {
   return (int)((st->step - st->ring_base) % st->ring_slots);
 }
+// The ring slot of the current iteration: given by the host (which mirrors
+// the step) so that the slot's loads need not wait for the state, else read
+// from the state.
+__device__ __forceinline__ int ring_slot_of(int32_t host, const DevState* st) {
+  return host >= 0 ? host : ring_slot(st);
+}
 
 struct SamplerArgs {
   DevGraph g;
@@ -142,6 +148,7 @@ struct PhiArgs {
   const void* noise;         // injected [slot][K] or null
   Key key;
   void* staging;             // [slot][ld]
+  int32_t ring_slot_host;    // the current ring slot when the host knows it, else -1
 };
 
 struct ScatterArgs {
@@ -154,6 +161,7 @@ struct ScatterArgs {
   const int32_t* nodes;
   int32_t count_host;
   const void* staging;
+  int32_t ring_slot_host;    // as PhiArgs
 };
 
 struct BetaArgs {
@@ -170,6 +178,7 @@ struct BetaArgs {
   const double* theta;      // [K][2]
   double epsilon;
   double* partials;         // [blocks][2K]
+  int32_t ring_slot_host;   // as PhiArgs
   int32_t blocks;
 };
 

@@ -913,7 +913,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
   const int32_t* nodes;
   const int32_t* nbrs;
   if (a.use_ring) {
-    const int rs = ring_slot(a.st);
+    const int rs = ring_slot_of(a.ring_slot_host, a.st);
     count = a.ring.meta[rs].node_count;
     nodes = a.ring.nodes + (int64_t)rs * a.ring.node_cap;
     nbrs = a.ring.nbrs + (int64_t)rs * a.ring.node_cap * a.nnb;
@@ -1064,7 +1064,7 @@ __global__ __launch_bounds__(256) void k_phi_finish(PhiArgs a) {
   int count;
   const int32_t* nodes;
   if (a.use_ring) {
-    const int rs = ring_slot(a.st);
+    const int rs = ring_slot_of(a.ring_slot_host, a.st);
     count = a.ring.meta[rs].node_count;
     nodes = a.ring.nodes + (int64_t)rs * a.ring.node_cap;
   } else {
@@ -1090,7 +1090,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   int count;
   const int32_t* nodes;
   if (a.use_ring) {
-    const int rs = ring_slot(a.st);
+    const int rs = ring_slot_of(a.ring_slot_host, a.st);
     count = a.ring.meta[rs].node_count;
     nodes = a.ring.nodes + (int64_t)rs * a.ring.node_cap;
   } else {
@@ -1131,7 +1131,7 @@ __device__ __forceinline__ void beta_body(const BetaArgs& a, const int bx) {
   const int32_t* edges;
   const uint8_t* ys;
   if (a.use_ring) {
-    const int rs = ring_slot(a.st);
+    const int rs = ring_slot_of(a.ring_slot_host, a.st);
     count = a.ring.meta[rs].mb_count;
     edges = a.ring.edges + (int64_t)rs * a.ring.edge_cap * 2;
     ys = a.ring.y + (int64_t)rs * a.ring.edge_cap;
