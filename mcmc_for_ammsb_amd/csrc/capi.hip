@@ -2104,6 +2104,17 @@ int ammsb_batch_create(int32_t n, const ammsb_options* opts, const ammsb_graph* 
     set_error("ammsb_batch_create: cannot create the sampler stream and its events");
     return fail(AMMSB_ERR_HIP);
   }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
+    cus = 256;
+  // update_beta workgroups per chain: the chains share one launch, so about
+  // as many workgroups in all as stay resident (each wavefront then takes
+  // several groups of edges, and fewer partial rows reach the theta step);
+  // a chain's own bound (its edge capacity) still applies.  64 chains: 41 us
+  // with each chain's own bound, 28 us at the resident count, 38 us at twice
+  // and 41 us at half of it.
+  const int32_t beta_per_chain =
+      std::max(1, (cus * beta_batch_blocks_per_cu(b->prec) + n - 1) / n);
   for (int32_t i = 0; i < n; ++i) {
     ammsb_ctx* c = b->chains[i];
     c->ring.slots = 2 * b->R;
@@ -2122,13 +2133,14 @@ int ammsb_batch_create(int32_t n, const ammsb_options* opts, const ammsb_graph* 
     pargs[i] = phi_args(c);
     bargs[i] = beta_args(c);
     targs[i] = theta_args(c);
+    bargs[i].blocks = targs[i].blocks = std::min(c->beta_blocks, beta_per_chain);
     beta_cfg[i] = batch_cfg_default(c->K, c->prec);
     b->shapes[i].cfg = default_cfg(c);
     b->shapes[i].npb = default_npb(c, b->shapes[i].cfg);
     if (c->mbtable_in_lds) b->mb_lds = std::max(b->mb_lds, (size_t)c->mbtable_cap * sizeof(int2));
     b->nb_lds = std::max(b->nb_lds, neighbor_lds_bytes(c->nbr_tcap));
     b->max_node_cap = std::max<int32_t>(b->max_node_cap, (int32_t)c->node_cap);
-    b->max_beta_blocks = std::max(b->max_beta_blocks, c->beta_blocks);
+    b->max_beta_blocks = std::max(b->max_beta_blocks, bargs[i].blocks);
     b->max_K = std::max(b->max_K, c->K);
   }
   int rc;
@@ -2156,9 +2168,6 @@ int ammsb_batch_create(int32_t n, const ammsb_options* opts, const ammsb_graph* 
       return fail(AMMSB_ERR_HIP);
     }
   if ((rc = batch_upload_shapes(b)) != AMMSB_OK) return fail(rc);
-  int cus = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
-    cus = 256;
   // persistent: as many workgroups as stay resident
   b->phi_grid = std::max(1, cus * update_phi_batch_blocks_per_cu(n, b->prec));
   // eight resident wavefronts per SIMD for the copy

@@ -378,6 +378,8 @@ hipError_t launch_scatter_batch(const PhiArgs* args, const int32_t* cfg, int n, 
 hipError_t launch_beta_batch(const BetaArgs* args, const int32_t* cfg, int n, int max_blocks,
                              int precision, hipStream_t s);
 hipError_t launch_theta_batch(const ThetaArgs* args, int n, int max_K, hipStream_t s);
+// Resident workgroups per CU of the batched update_beta.
+int beta_batch_blocks_per_cu(int precision);
 // Batched update_phi shape validity for K / precision.
 bool batch_cfg_valid(int cfg, int32_t K, int precision);
 int batch_cfg_default(int32_t K, int precision);  // -1: K too large for batching

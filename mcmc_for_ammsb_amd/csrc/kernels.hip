@@ -3073,6 +3073,16 @@ hipError_t launch_scatter_batch(const PhiArgs* args, const int32_t* cfg, int n, 
   return hipGetLastError();
 }
 
+int beta_batch_blocks_per_cu(int precision) {
+  int nb = 0;
+  const hipError_t e =
+      precision == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                           &nb, (const void*)k_beta_batch<double>, kBetaWaves * 64, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                           &nb, (const void*)k_beta_batch<float>, kBetaWaves * 64, 0);
+  return e == hipSuccess && nb > 0 ? nb : 1;
+}
+
 hipError_t launch_beta_batch(const BetaArgs* args, const int32_t* cfg, int n, int max_blocks,
                              int precision, hipStream_t s) {
   if (n < 1 || max_blocks < 1) return hipErrorInvalidValue;
