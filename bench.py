@@ -681,8 +681,10 @@ def run_batch(args):
                          for sh in shapes]
     views = [b.chain(i) for i in range(len(b))]
     res = {}
-    for mode, graph, in_order in (("graph", True, True), ("graph_refill_overlapped", True, False),
-                                  ("direct", False, True)):
+    # "graph" is the default configuration: hipGraph replay, refills
+    # overlapped on a second stream
+    for mode, graph, in_order in (("graph", True, False), ("graph_refill_in_order", True, True),
+                                  ("direct", False, False)):
         b.use_graph(graph)
         b.refill_in_order(in_order)
         b.step(2)

@@ -1825,7 +1825,7 @@ struct ammsb_batch {
   int32_t max_node_cap = 0, max_beta_blocks = 0, max_K = 0;
   int32_t phi_grid = 1, scatter_grid = 1;
   bool use_graph = true;
-  bool refill_in_order = true;  // sampler refills on the main stream (else a second stream)
+  bool refill_in_order = false;  // sampler refills on the main stream (else a second stream)
   hipGraph_t graph = nullptr;  // one iteration (samplers excluded)
   hipGraphExec_t exec = nullptr;
 };

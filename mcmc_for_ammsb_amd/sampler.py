@@ -327,7 +327,7 @@ class MCMCBatch:
 
     def refill_in_order(self, enable=True):
         """Draw the ring's next 16 iterations on the batch's own stream (in
-        order) instead of overlapping them on a second stream."""
+        order) instead of overlapping them on a second stream (the default)."""
         _lib.check(_lib.lib.ammsb_batch_refill_in_order(self._h, 1 if enable else 0))
 
     def get_shape(self, i):

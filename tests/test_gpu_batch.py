@@ -156,12 +156,12 @@ def test_graph_replay_equals_direct_launches():
 
 
 def test_refill_modes_same_trajectory():
-    """Sampler refills in stream order (default) and overlapped on a second
-    stream draw the same iterations: bit-identical states across refills."""
+    """Sampler refills in stream order and overlapped on a second stream (the
+    default) draw the same iterations: bit-identical states across refills."""
     print("batch refill in order vs overlapped", flush=True)
     a = _batch("fp32", CHAINS)[0]
     b = _batch("fp32", CHAINS)[0]
-    b.refill_in_order(False)
+    a.refill_in_order(True)
     for steps in (3, 17, 20):
         a.step(steps)
         b.step(steps)
