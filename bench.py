@@ -324,9 +324,10 @@ def cpu_leg(orc, opts, start_step, threads, warmup, timed, with_ppx):
     }
 
 
-def cpu_baseline(g, s, w, pi32, theta, start_step, args):
+def cpu_baseline(g, s, w, pi32, theta, start_step, args, primary):
     """The oracle loop on the host cores, FP32 and FP64, threaded (the
-    distributed reference's OpenMP loops) and on one thread."""
+    distributed reference's OpenMP loops) and on one thread; `value` in the
+    workload's primary precision, the other beside it."""
     from tests.oracle_lib import Oracle
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     out = {}
@@ -346,9 +347,10 @@ def cpu_baseline(g, s, w, pi32, theta, start_step, args):
         log(f"  1 thread: {seq['value']:.3e} node*K/s ({time.time() - t:.1f}s)")
         out[prec] = {"threads": omp, "one_thread": seq}
         del orc
-    cb = dict(out["fp32"]["threads"])
-    cb.update({"kind": "port", "precision": "fp32",
-               "fp64": out["fp64"]["threads"],
+    other = "fp64" if primary == "fp32" else "fp32"
+    cb = dict(out[primary]["threads"])
+    cb.update({"kind": "port", "precision": primary,
+               other: out[other]["threads"],
                "one_thread": {"fp32": out["fp32"]["one_thread"],
                               "fp64": out["fp64"]["one_thread"]},
                "host": host_info()})
@@ -446,7 +448,7 @@ def run_throughput(args, ttt_only=False):
     rank = int(os.environ.get("RANK", "0"))
     if rank == 0 and keep is not None:
         pi32, theta, step = keep
-        out["cpu_baseline"] = cpu_baseline(g, s, w, pi32, theta, step, args)
+        out["cpu_baseline"] = cpu_baseline(g, s, w, pi32, theta, step, args, primary)
         del pi32
     if rank == 0:
         print(json.dumps(out), flush=True)

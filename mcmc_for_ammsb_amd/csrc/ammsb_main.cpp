@@ -59,7 +59,7 @@ int main(int argc, char** argv) {
   if (!edge_list_class(input.input_class)) {
     fprintf(stderr,
             "IO error: input class '%s' is not read by this driver (edge lists: relativity, "
-            "rc, rz, rcz; netscience XML through the Python front end)\n",
+            "rc, rz, rcz)\n",
             input.input_class);
     return 33;
   }

@@ -81,40 +81,20 @@ class Graph:
         return g
 
     @classmethod
-    def from_netscience_xml(cls, path):
-        """The reference's "netscience" input class (netscience.cc:28-134):
-        N = the number of <node> elements of the first nodeclass, one edge per
-        <link source= target=> of the first network."""
-        import xml.etree.ElementTree as ET  # parses text only
-        root = ET.parse(str(path)).getroot()
-        if root.tag != "DynamicNetwork":
-            raise ValueError("Cannot get 'DynamicNetwork'")
-        meta = root.find("MetaNetwork")
-        if meta is None:
-            raise ValueError("Cannot get 'MetaNetwork'")
-        nodeclass = meta.find("nodes/nodeclass")
-        network = meta.find("networks/network")
-        if nodeclass is None or network is None:
-            raise ValueError("Cannot get 'nodes/nodeclass' or 'networks/network'")
-        ids = [int(n.get("id")) for n in nodeclass.findall("node")]
-        N = len(set(ids))
-        links = [(int(l.get("target")), int(l.get("source"))) for l in network.findall("link")]
-        e = np.array(links, dtype=np.int64).reshape(-1, 2)
-        if len(e) and (e.min() < 0 or e.max() >= N):
-            raise ValueError("link id outside the node range")
-        return cls.from_edges(N, e)
-
-    @classmethod
     def from_options(cls, options):
-        """The reference's DataFactory (data_factory.cc:7-60): input_class
-        relativity / rc / rz / rcz (contiguous and compressed switches) or
-        netscience, reading options.input_filename."""
+        """The reference's DataFactory (data_factory.cc:7-60) for its edge-list
+        classes: input_class relativity / rc / rz / rcz (contiguous and
+        compressed switches), reading options.input_filename.  The XML
+        (netscience) and sparsehash classes are dataset parsers outside the
+        sampler and are refused."""
         klass = options.input_class
         contiguous = options.input_contiguous or klass in ("rc", "rcz")
         if klass in ("relativity", "rc", "rz", "rcz"):
             return cls.from_edge_list_file(options.input_filename, contiguous=contiguous)
         if klass == "netscience":
-            return cls.from_netscience_xml(options.input_filename or "datasets/netscience.xml")
+            raise NotImplementedError(
+                "input class 'netscience' is an XML dataset parser, outside this library; "
+                "convert the graph to an edge list")
         if klass in ("sparsehash", "gz", "preprocessed"):
             raise NotImplementedError(
                 f"input class {klass!r} is google sparsehash's serialized table format, which "

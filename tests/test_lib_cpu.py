@@ -239,15 +239,9 @@ def test_graph_from_edges_and_file(tmp_path):
         M.Graph.from_edge_list_file(str(tmp_path / "missing.txt"))
 
 
-def test_graph_from_options_and_netscience(tmp_path):
-    xml = tmp_path / "ns.xml"
-    xml.write_text('<DynamicNetwork><MetaNetwork><nodes><nodeclass id="agent" type="agent">'
-                   '<node id="0" title="A"/><node id="1" title="B"/><node id="2" title="C"/>'
-                   '</nodeclass></nodes><networks><network id="agent x agent" isDirected="false">'
-                   '<link source="1" target="0" value="2.5"/><link source="2" target="1"/>'
-                   '</network></networks></MetaNetwork></DynamicNetwork>')
-    g = M.Graph.from_options(Options.parse(["--mcmc.input.class", "netscience", "-f", str(xml)]))
-    assert g.N == 3 and g.num_edges == 2 and g.linked(0, 1) and g.linked(1, 2)
+def test_graph_from_options(tmp_path):
+    with pytest.raises(NotImplementedError):
+        M.Graph.from_options(Options.parse(["--mcmc.input.class", "netscience", "-f", "x.xml"]))
     f = tmp_path / "e.txt"
     f.write_text("#\n#\n#\n#\n0 1\n1 2\n")
     g = M.Graph.from_options(Options.parse(["-c", "rc", "-f", str(f)]))
