@@ -329,12 +329,6 @@ int ammsb_get_totals(ammsb_ctx *ctx, int64_t *nodes_updated,
  *                wider than 2 KB streams whole rows per wavefront against
  *                the source row held in LDS; 0: the K-chunked kernels above
  *                (defaults: forward 1; adjoint 1 for FP64, 0 for FP32)
- *   "phi_split"  1: update_phi as a gather kernel + a phi-step kernel
- *                instead of the fused step + scatter (default 0)
- *   "async_ring" 1: the minibatch/neighbour samplers draw the next 16
- *                iterations on a second stream while the current 16 run
- *                (default 0: on one MI355X the overlap gains nothing,
- *                the samplers take CUs from update_phi)
  * Set phi_waves first; the (waves, unroll) pair is validated when
  * phi_unroll is set.  Results do not depend on the shape beyond
  * summation order (the log-likelihood shapes not even on that).  Defaults:

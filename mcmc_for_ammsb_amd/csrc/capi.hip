@@ -85,18 +85,6 @@ struct ammsb_ctx {
   ammsb_batch* owner = nullptr;     // batch that owns this chain, if any
   int32_t max_ring_slots = 16;
   int64_t ring_gen = 0;             // bumped by every single-context ring refill
-  // Sampler refills ahead on a second stream (single contexts): the ring is
-  // two halves of R slots; block k = iterations [aring_B + kR, +R) lives in
-  // half k mod 2, and block k+1 is drawn while block k computes.
-  bool async_ring = false;
-  int32_t aR = 0;
-  hipStream_t fill_stream = nullptr;
-  hipEvent_t ev_fill[2] = {nullptr, nullptr};
-  hipEvent_t ev_done[2] = {nullptr, nullptr};
-  SamplerArgs* d_sargs1 = nullptr;    // this context's sampler arguments on the device
-  NeighborArgs* d_nargs1 = nullptr;
-  bool aring_ok = false;
-  int64_t aring_B = 0, aring_hi = 0, aring_gen = -1;
   Key key{};
   DevHyper hp{};
   // device graph
@@ -130,7 +118,6 @@ struct ammsb_ctx {
   int64_t mbtable_cap = 0;
   int32_t mbtable_in_lds = 0;
   int32_t phi_waves = kPhiWavesDefault, phi_unroll = kPhiUnrollDefault;
-  int32_t phi_split = 0;  // 1: gather + finish kernels instead of fused step + scatter
   // K-chunked log-likelihood launch shapes [0] forward, [1] adjoint
   int32_t ll_pairs[2] = {0, 0}, ll_prefetch[2] = {0, 0};
   int32_t ll_fwd_rows = 1;  // forward of wide rows: whole rows per wavefront (else K-chunked)
@@ -428,7 +415,6 @@ ThetaArgs theta_args(const ammsb_ctx* c) {
 // device's step_count) into ring slots 0..fills-1
 int fill_ring(ammsb_ctx* c, int64_t t0, int32_t fills) {
   c->ring_gen++;
-  if (c->fill_stream) HIP_TRY(hipStreamSynchronize(c->fill_stream));  // no refill in flight
   {
     PhaseScope ps(c, PH_MINIBATCH);
     HIP_TRY(launch_minibatch(sampler_args(c, t0), fills, c->stream));
@@ -440,79 +426,20 @@ int fill_ring(ammsb_ctx* c, int64_t t0, int32_t fills) {
   return AMMSB_OK;
 }
 
-// ---- sampler refills ahead (async_ring): block k of R iterations into half
-// k mod 2 of the ring
-int aring_fill(ammsb_ctx* c, int64_t k, hipStream_t s) {
-  const int32_t R = c->aR;
-  const size_t mb_lds = c->mbtable_in_lds ? (size_t)c->mbtable_cap * sizeof(int2) : 0;
-  HIP_TRY(launch_minibatch_batch(c->d_sargs1, 1, c->aring_B + k * R, (int)(k % 2) * R, R, 2 * R,
-                                 mb_lds, s));
-  HIP_TRY(launch_neighbors_batch(c->d_nargs1, 1, (int)(k % 2) * R, R, 2 * R, (int)c->node_cap,
-                                 neighbor_lds_bytes(c->nbr_tcap), s));
-  return AMMSB_OK;
-}
-
-// Before iteration step_count: the ring set up (block 0 drawn in stream
-// order on the main stream) when it does not hold this step; at a block's
-// first iteration, wait for its draws and start drawing the next block on
-// the sampler stream, after the iterations that used its half.
-int aring_before(ammsb_ctx* c) {
-  const int64_t S = c->step_count;
-  const int32_t R = c->aR;
-  if (!c->aring_ok || c->aring_gen != c->ring_gen || S < c->aring_hi - 2 * R ||
-      S >= c->aring_hi) {
-    HIP_TRY(hipStreamSynchronize(c->fill_stream));
-    c->aring_B = S;
-    HIP_TRY(launch_set_ring(c->d_st, S, 2 * R, c->stream));
-    const int rc = aring_fill(c, 0, c->stream);
-    if (rc != AMMSB_OK) return rc;
-    HIP_TRY(hipEventRecord(c->ev_fill[0], c->stream));
-    c->aring_hi = S + R;
-    c->aring_ok = true;
-    c->aring_gen = c->ring_gen;
-    c->ring_valid = false;  // the synchronous ring bookkeeping no longer applies
-  }
-  const int64_t k = (S - c->aring_B) / R, pos = (S - c->aring_B) % R;
-  if (pos == 0) {
-    HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_fill[k % 2], 0));
-    if (c->aring_hi == c->aring_B + (k + 1) * R) {
-      HIP_TRY(hipStreamWaitEvent(c->fill_stream, c->ev_done[(k + 1) % 2], 0));
-      const int rc = aring_fill(c, k + 1, c->fill_stream);
-      if (rc != AMMSB_OK) return rc;
-      HIP_TRY(hipEventRecord(c->ev_fill[(k + 1) % 2], c->fill_stream));
-      c->aring_hi += R;
-    }
-  }
-  return AMMSB_OK;
-}
-
-int aring_after(ammsb_ctx* c) {
-  const int64_t S = c->step_count;
-  const int64_t k = (S - c->aring_B) / c->aR, pos = (S - c->aring_B) % c->aR;
-  if (pos == c->aR - 1) HIP_TRY(hipEventRecord(c->ev_done[k % 2], c->stream));
-  return AMMSB_OK;
-}
-
 // ---- one SGRLD iteration on the device (mcmc_sampler_stochastic.cc:131-174)
-// The minibatch and neighbour draws come from the ring: drawn ahead on the
-// sampler stream (async_ring), or refilled on this stream every ring.slots
-// iterations.
+// The minibatch and neighbour draws come from the ring, refilled on this
+// stream every ring.slots iterations.
 int enqueue_iteration(ammsb_ctx* c) {
-  if (c->async_ring) {
-    const int rc = aring_before(c);
-    if (rc != AMMSB_OK) return rc;
-  } else if (!c->ring_valid || c->step_count < c->ring_base ||
-             c->step_count >= c->ring_base + c->ring.slots) {
+  if (!c->ring_valid || c->step_count < c->ring_base ||
+      c->step_count >= c->ring_base + c->ring.slots) {
     int rc = fill_ring(c, c->step_count, c->ring.slots);
     if (rc != AMMSB_OK) return rc;
     c->ring_base = c->step_count;
     c->ring_valid = true;
   }
-  const bool split = c->phi_split && c->phi_waves == kPhiWavesDefault &&
-                     c->phi_unroll == kPhiUnrollDefault;
   // the ring holds [ring_base, ring_base + slots) here (the sampler set the
   // device's ring_base to the same step), so the slot is known on the host
-  const int32_t rs = c->async_ring ? -1 : (int32_t)(c->step_count - c->ring_base);
+  const int32_t rs = (int32_t)(c->step_count - c->ring_base);
   PhiArgs pa = phi_args(c);
   pa.ring_slot_host = rs;
   ScatterArgs sa = scatter_args(c);
@@ -522,21 +449,17 @@ int enqueue_iteration(ammsb_ctx* c) {
   {
     PhaseScope ps(c, PH_UPDATE_PHI);
     HIP_TRY(launch_update_phi(pa, c->prec, (int32_t)c->node_cap, c->stream, c->phi_waves,
-                              c->phi_unroll, split));
+                              c->phi_unroll));
   }
   {
     PhaseScope ps(c, PH_UPDATE_PI);
-    if (split)
-      HIP_TRY(launch_phi_finish(pa, c->prec, (int32_t)c->node_cap, c->stream));
-    else
-      HIP_TRY(launch_scatter(sa, c->prec, (int32_t)c->node_cap, c->stream));
+    HIP_TRY(launch_scatter(sa, c->prec, (int32_t)c->node_cap, c->stream));
   }
   {
     PhaseScope ps(c, PH_UPDATE_BETA);
     HIP_TRY(launch_beta(ba, c->prec, c->stream));
     HIP_TRY(launch_theta(theta_args(c), c->stream));
   }
-  if (c->async_ring) return aring_after(c);
   return AMMSB_OK;
 }
 
@@ -632,16 +555,6 @@ void free_ctx(ammsb_ctx* c) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
-  if (c->fill_stream) {
-    (void)hipStreamSynchronize(c->fill_stream);
-    (void)hipStreamDestroy(c->fill_stream);
-  }
-  for (int h = 0; h < 2; ++h) {
-    if (c->ev_fill[h]) (void)hipEventDestroy(c->ev_fill[h]);
-    if (c->ev_done[h]) (void)hipEventDestroy(c->ev_done[h]);
-  }
-  if (c->d_sargs1) (void)hipFree(c->d_sargs1);
-  if (c->d_nargs1) (void)hipFree(c->d_nargs1);
   if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -925,27 +838,6 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
   c->ppx_blocks = (int32_t)std::max<int64_t>(
       1, std::min<int64_t>(2048, (c->H + 4 * rpw * 8 - 1) / (4 * rpw * 8)));
   ALLOC_TRY(c->d_ppx_partials, (size_t)c->ppx_blocks * 4);
-  if (!c->owner && c->ring.slots >= 2 && c->nbr_use_lds) {
-    // draw each next block of iterations on a sampler stream (see aring_before)
-    c->aR = c->ring.slots / 2;
-    c->ring.slots = 2 * c->aR;
-    HIP_TRY(hipStreamCreateWithFlags(&c->fill_stream, hipStreamNonBlocking));
-    for (int h = 0; h < 2; ++h) {
-      HIP_TRY(hipEventCreateWithFlags(&c->ev_fill[h], hipEventDisableTiming));
-      HIP_TRY(hipEventCreateWithFlags(&c->ev_done[h], hipEventDisableTiming));
-    }
-    ALLOC_TRY(c->d_sargs1, 1);
-    ALLOC_TRY(c->d_nargs1, 1);
-    const SamplerArgs sa = sampler_args(c, -1);
-    const NeighborArgs na = neighbor_args(c);
-    HIP_TRY(hipMemcpy(c->d_sargs1, &sa, sizeof sa, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_nargs1, &na, sizeof na, hipMemcpyHostToDevice));
-    // off by default: measured on one MI355X, the overlap is a wash (N=1M
-    // K=256 FP64: 0.261 vs 0.262 ms per step; N=4M K=1024 FP32: 0.654 vs
-    // 0.650) because the samplers take CUs from update_phi;
-    // set_tuning("async_ring", 1) enables it
-    c->async_ring = false;
-  }
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (o.verbose) {
     // learner.cc:105-120, mcmc_sampler_stochastic.cc:84-91
@@ -1328,7 +1220,6 @@ int ammsb_update_phi(ammsb_ctx* c, const int32_t* nodes, int64_t n_nodes, const 
     e = hipMemcpy(d_noise, noise, n_nodes * c->K * c->fsize, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = launch_label_nbrs(dev_graph(c), d_nodes, (int32_t)n_nodes, d_nbrs, (int32_t)n_nb, c->stream);
-  bool split = false;
   if (e == hipSuccess) {
     PhiArgs pa{};
     pa.pi = c->d_pi;
@@ -1348,12 +1239,9 @@ int ammsb_update_phi(ammsb_ctx* c, const int32_t* nodes, int64_t n_nodes, const 
     pa.noise = d_noise;
     pa.key = c->key;
     pa.staging = d_stage;
-    split = c->phi_split && c->phi_waves == kPhiWavesDefault && c->phi_unroll == kPhiUnrollDefault;
-    e = launch_update_phi(pa, c->prec, (int32_t)n_nodes, c->stream, c->phi_waves, c->phi_unroll,
-                          split);
-    if (e == hipSuccess && split) e = launch_phi_finish(pa, c->prec, (int32_t)n_nodes, c->stream);
+    e = launch_update_phi(pa, c->prec, (int32_t)n_nodes, c->stream, c->phi_waves, c->phi_unroll);
   }
-  if (e == hipSuccess && !split) {
+  if (e == hipSuccess) {
     ScatterArgs sc{};
     sc.pi = c->d_pi;
     sc.ld = c->ld;
@@ -1491,23 +1379,6 @@ int ammsb_get_totals(ammsb_ctx* c, int64_t* nodes_updated, int64_t* minibatch_ed
 int ammsb_set_tuning(ammsb_ctx* c, const char* name, int64_t value) {
   if (!c || !name) return AMMSB_ERR_ARG;
   const std::string n(name);
-  if (n == "phi_split") {
-    if (value != 0 && value != 1) return AMMSB_ERR_ARG;
-    c->phi_split = (int32_t)value;
-    return AMMSB_OK;
-  }
-  if (n == "async_ring") {
-    if (value != 0 && value != 1) return AMMSB_ERR_ARG;
-    if (value == 1 && !c->fill_stream) {
-      set_error("ammsb_set_tuning: this context has no sampler stream");
-      return AMMSB_ERR_ARG;
-    }
-    if (c->fill_stream) HIP_TRY(hipStreamSynchronize(c->fill_stream));
-    c->async_ring = value == 1;
-    c->aring_ok = false;
-    c->ring_valid = false;
-    return AMMSB_OK;
-  }
   if (n == "ll_fwd_rows" || n == "ll_bwd_rows") {
     if (value != 0 && value != 1) return AMMSB_ERR_ARG;
     (n == "ll_fwd_rows" ? c->ll_fwd_rows : c->ll_bwd_rows) = (int32_t)value;
@@ -1554,10 +1425,6 @@ int ammsb_get_tuning(ammsb_ctx* c, const char* name, int64_t* value) {
     *value = c->phi_waves;
   else if (n == "phi_unroll")
     *value = c->phi_unroll;
-  else if (n == "phi_split")
-    *value = c->phi_split;
-  else if (n == "async_ring")
-    *value = c->async_ring ? 1 : 0;
   else if (n == "ll_fwd_pairs")
     *value = c->ll_pairs[0];
   else if (n == "ll_fwd_prefetch")
@@ -1611,11 +1478,9 @@ int ammsb_probe_gather(ammsb_ctx* c, int32_t reps, double* seconds, int64_t* byt
     return AMMSB_ERR_STATE;
   }
   HIP_TRY(hipSetDevice(c->opt.device));
-  if (c->fill_stream) HIP_TRY(hipStreamSynchronize(c->fill_stream));
   // the draws of iteration step_count into ring slot 0; the ring is refilled
   // before the next iteration runs
   c->ring_valid = false;
-  c->aring_ok = false;
   int rc = fill_ring(c, c->step_count, 1);
   if (rc != AMMSB_OK) return rc;
   IterMeta mt;

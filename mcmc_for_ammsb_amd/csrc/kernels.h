@@ -229,11 +229,10 @@ constexpr int kNbrRowLds = 256;
 inline size_t neighbor_lds_bytes(int32_t tcap) {
   return (size_t)4 * ((size_t)tcap * 8 + (size_t)kNbrRowLds * 4);
 }
-// split: the gradient sums are staged and launch_phi_finish applies the phi
-// step and writes pi (in place of the fused step + launch_scatter)
+// update_phi + the phi step of grid_nodes minibatch nodes, new rows staged
+// (launch_scatter publishes them)
 hipError_t launch_update_phi(const PhiArgs& a, int precision, int32_t grid_nodes,
-                             hipStream_t s, int waves, int unroll, bool split = false);
-hipError_t launch_phi_finish(const PhiArgs& a, int precision, int32_t grid_nodes, hipStream_t s);
+                             hipStream_t s, int waves, int unroll);
 // Launch shape (wavefronts per node, neighbour rows in flight) available for K.
 bool update_phi_shape_supported(int32_t K, int precision, int waves, int unroll);
 constexpr int kPhiWavesDefault = 4;

@@ -897,10 +897,9 @@ __device__ __forceinline__ double phi_step_size(const PhiArgs& a) {
   return a.eps_override >= 0.0 ? a.eps_override : a.st->eps_t;
 }
 
-// EPI: the phi step is applied here and the new row staged (k_scatter
-// publishes it); else the gradient sums are staged and k_phi_finish applies
-// the step (fewer registers here: the Philox/Box-Muller epilogue is gone).
-template <typename F, int LPR, int CH, int WAVES, int U, bool EPI = true>
+// The phi step is applied here and the new row staged; k_scatter publishes
+// it once every gather of the iteration is done.
+template <typename F, int LPR, int CH, int WAVES, int U>
 __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
   constexpr int V = VT<F>::V;
   constexpr int EPL = V * CH;
@@ -942,8 +941,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
     if (r < a.nnb) ids = nb_src[r];
   }
   const F phi_sum = prow[K];
-  const int64_t t = EPI ? a.st->step : 0;
-  const double eps_t = EPI ? phi_step_size(a) : 0.0;
+  const int64_t t = a.st->step;
+  const double eps_t = phi_step_size(a);
 
   F pii[EPL], f1[EPL], acc[EPL];
   load_row<F, LPR, CH>(prow, K, lg, pii);
@@ -1037,50 +1036,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
   }
   if (g != 0) return;
 
-  if constexpr (EPI) {
-    phi_epilogue<F, LPR, CH>(a, slot, node, phi_sum, pii, acc, lg,
-                             (F*)a.staging + (int64_t)slot * a.ld, t, eps_t);
-  } else {
-    F* stg = (F*)a.staging + (int64_t)slot * a.ld;
-#pragma unroll
-    for (int c = 0; c < CH; ++c)
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        const int k = (c * LPR + lg) * V + v;
-        if (k < K) stg[k] = acc[c * V + v];
-      }
-  }
-}
-
-// phi step + update_pi from the staged gradient sums: one row group of LPR
-// lanes per minibatch node, the node's row rewritten in pi.  Runs after
-// every neighbour-row read of the iteration (previous launch), as the
-// reference refreshes pi after the node loop (mcmc_sampler_stochastic.cc:162-166).
-template <typename F, int LPR, int CH>
-__global__ __launch_bounds__(256) void k_phi_finish(PhiArgs a) {
-  constexpr int V = VT<F>::V;
-  constexpr int EPL = V * CH;
-  constexpr int RPW = 64 / LPR;
-  int count;
-  const int32_t* nodes;
-  if (a.use_ring) {
-    const int rs = ring_slot_of(a.ring_slot_host, a.st);
-    count = a.ring.meta[rs].node_count;
-    nodes = a.ring.nodes + (int64_t)rs * a.ring.node_cap;
-  } else {
-    count = a.count_host;
-    nodes = a.nodes;
-  }
-  const int lane = threadIdx.x & 63, g = lane / LPR, lg = lane % LPR;
-  const int slot = (blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW + g;
-  if (slot >= count) return;
-  const int32_t node = nodes[slot];
-  F* prow = (F*)a.pi + (int64_t)node * a.ld;
-  F pii[EPL], acc[EPL];
-  load_row<F, LPR, CH>(prow, a.K, lg, pii);
-  load_row<F, LPR, CH>((const F*)a.staging + (int64_t)slot * a.ld, a.K, lg, acc);
-  phi_epilogue<F, LPR, CH>(a, slot, node, prow[a.K], pii, acc, lg, prow, a.st->step,
-                           phi_step_size(a));
+  phi_epilogue<F, LPR, CH>(a, slot, node, phi_sum, pii, acc, lg,
+                           (F*)a.staging + (int64_t)slot * a.ld, t, eps_t);
 }
 
 // Publish staged rows: pi[nodes[slot]] = staging[slot] (K+1 elements).
@@ -1626,14 +1583,7 @@ bool phi_shape_ok(int lpr, int ch, int waves, int unroll) {
 }
 
 template <typename F, int L, int C>
-hipError_t phi_launch(const PhiArgs& a, int grid, hipStream_t s, int waves, int unroll,
-                      bool split) {
-  if (split) {  // gather of the default shape; k_phi_finish follows
-    if (waves != kPhiWaves || unroll != kPhiUnroll) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_update_phi<F, L, C, kPhiWaves, kPhiUnroll, false>), dim3(grid),
-                       dim3(kPhiWaves * 64), 0, s, a);
-    return hipGetLastError();
-  }
+hipError_t phi_launch(const PhiArgs& a, int grid, hipStream_t s, int waves, int unroll) {
 #define V(W, U)                                                                        \
   if (waves == W && unroll == U) {                                                     \
     hipLaunchKernelGGL((k_update_phi<F, L, C, W, U>), dim3(grid), dim3(W * 64), 0, s, a); \
@@ -1655,7 +1605,7 @@ bool update_phi_shape_supported(int32_t K, int precision, int waves, int unroll)
 }
 
 hipError_t launch_update_phi(const PhiArgs& a, int precision, int32_t grid_nodes,
-                             hipStream_t s, int waves, int unroll, bool split) {
+                             hipStream_t s, int waves, int unroll) {
   int lpr, ch;
   if (!row_config(a.K, precision, &lpr, &ch)) return hipErrorInvalidValue;
   if (!phi_shape_ok(lpr, ch, waves, unroll)) return hipErrorInvalidValue;
@@ -1663,27 +1613,8 @@ hipError_t launch_update_phi(const PhiArgs& a, int precision, int32_t grid_nodes
 #define X(L, C)                                                               \
   if (lpr == L && ch == C) {                                                  \
     if (precision == 0)                                                       \
-      return phi_launch<double, L, C>(a, grid_nodes, s, waves, unroll, split); \
-    return phi_launch<float, L, C>(a, grid_nodes, s, waves, unroll, split);    \
-  }
-  AMMSB_ROW_CONFIGS(X)
-#undef X
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_phi_finish(const PhiArgs& a, int precision, int32_t grid_nodes, hipStream_t s) {
-  int lpr, ch;
-  if (!row_config(a.K, precision, &lpr, &ch)) return hipErrorInvalidValue;
-  if (grid_nodes <= 0) return hipSuccess;
-  const int per_block = 4 * (64 / lpr);
-  const dim3 grid((grid_nodes + per_block - 1) / per_block);
-#define X(L, C)                                                                  \
-  if (lpr == L && ch == C) {                                                     \
-    if (precision == 0)                                                          \
-      hipLaunchKernelGGL((k_phi_finish<double, L, C>), grid, dim3(256), 0, s, a); \
-    else                                                                         \
-      hipLaunchKernelGGL((k_phi_finish<float, L, C>), grid, dim3(256), 0, s, a);  \
-    return hipGetLastError();                                                    \
+      return phi_launch<double, L, C>(a, grid_nodes, s, waves, unroll); \
+    return phi_launch<float, L, C>(a, grid_nodes, s, waves, unroll);    \
   }
   AMMSB_ROW_CONFIGS(X)
 #undef X

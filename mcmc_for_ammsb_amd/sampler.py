@@ -19,8 +19,8 @@ PHASES = ("outer", "perplexity", "sample_mini_batch", "nodes_in_mini_batch",
 
 
 # set in this order: each launch-shape pair is validated on its second key
-TUNING_KEYS = ("phi_waves", "phi_unroll", "phi_split", "async_ring", "ll_fwd_rows",
-               "ll_bwd_rows", "ll_fwd_pairs", "ll_fwd_prefetch", "ll_bwd_pairs",
+TUNING_KEYS = ("phi_waves", "phi_unroll", "ll_fwd_rows", "ll_bwd_rows", "ll_fwd_pairs",
+               "ll_fwd_prefetch", "ll_bwd_pairs",
                "ll_bwd_prefetch")
 
 
