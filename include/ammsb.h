@@ -332,8 +332,10 @@ int ammsb_get_totals(ammsb_ctx *ctx, int64_t *nodes_updated,
  * Set phi_waves first; the (waves, unroll) pair is validated when
  * phi_unroll is set.  Results do not depend on the shape beyond
  * summation order (the log-likelihood shapes not even on that).  Defaults:
- * update_phi (4, 1); log-likelihood FP64 (4,0) both ways, FP32 forward
- * (8,0), adjoint (2,1).
+ * update_phi (8, 2) where that shape exists for the row (a row held by one
+ * wavefront in 1, 2, 4 or 8 vectors per lane, e.g. FP32 K 1024), else
+ * (4, 1); log-likelihood FP64 (4,0) both ways, FP32 forward (8,0), adjoint
+ * (2,1).
  * ammsb_get_tuning also reads "sampler_lds" (read-only): bit 0 set when the
  * minibatch sampler's de-dup table lives in LDS, bit 1 when the neighbour
  * sampler's tables do (else they live in HBM); and "ring_slots"

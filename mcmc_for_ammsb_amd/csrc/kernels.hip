@@ -108,6 +108,15 @@ __device__ __forceinline__ void store_vec_nt<float>(float* p, const float* in) {
   __builtin_nontemporal_store(v, reinterpret_cast<st_f4*>(p));
 }
 
+// beta[k] in Float, 0 past the row (K >= 1).  The load is unconditional,
+// from a clamped index: a load under `k < K ? beta[k] : 0` becomes a branch
+// with its own wait, and a row's worth of them run one after another.
+template <typename F>
+__device__ __forceinline__ F beta_at(const double* beta, int k, int K) {
+  const double b = beta[k < K ? k : K - 1];
+  return k < K ? (F)b : (F)0;
+}
+
 // Streaming (non-temporal) 16-byte loads, for rows read once per use.
 typedef double nt_d2 __attribute__((ext_vector_type(2)));
 typedef float nt_f4 __attribute__((ext_vector_type(4)));
@@ -905,8 +914,16 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
   constexpr int EPL = V * CH;
   constexpr int RPW = 64 / LPR;
   constexpr int STRIDE = WAVES * RPW;
-  __shared__ F s_acc[(WAVES > 1 ? WAVES - 1 : 1) * EPL * LPR];
-  __shared__ int32_t s_nb[kPhiNbLds];
+  // The neighbour ids staged in LDS (more than 64*WAVES neighbours) during
+  // the row loop, then the wavefronts' partial sums (EPL per lane of a row
+  // group and the scalar sum C): one buffer, a barrier between the uses.
+  constexpr int kAccWords = WAVES * (EPL + 1) * LPR * (int)sizeof(F) / 4;
+  __shared__ int32_t s_buf[kAccWords > kPhiNbLds ? kAccWords : kPhiNbLds];
+  int32_t* const s_nb = s_buf;
+  F* const s_acc = reinterpret_cast<F*>(s_buf);
+  __shared__ double s_sum[WAVES];
+  __shared__ F s_pii[EPL * LPR];  // the node's pi row, for the phi step
+  __shared__ F s_f[EPL * LPR];    // f_k = beta_k - eps (0 past the row)
   const int slot = blockIdx.x;
   int count;
   const int32_t* nodes;
@@ -944,18 +961,46 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
   const int64_t t = a.st->step;
   const double eps_t = phi_step_size(a);
 
-  F pii[EPL], f1[EPL], acc[EPL];
-  load_row<F, LPR, CH>(prow, K, lg, pii);
+  // The gradient is regrouped so that the row loop holds three arrays:
+  //   x_bk = s_b f_k pi_bk + e_b  (link: s = 1, e = eps; non-link: s = -1,
+  //   e = 1 - eps; f_k = beta_k - eps),
+  //   Z_b = sum_k pi_ak x_bk = s_b P_b + e_b S,  P_b = sum_k q_k pi_bk,
+  //   q_k = pi_ak f_k,  S = sum_k pi_ak,
+  //   grads_k = sum_b (x_bk / Z_b - 1) = f_k acc_k + C,
+  //   acc_k = sum_b s_b pi_bk / Z_b,  C = sum_b (e_b / Z_b - 1),
+  // the same sums as update_phi_node's (mcmc_sampler_stochastic_distr.cc:
+  // 1184-1205) in another order.
+  F q[EPL], acc[EPL];
+  F Sp = (F)0;
+  {
+    F pii[EPL];
+    load_row<F, LPR, CH>(prow, K, lg, pii);
+    // f_k into LDS, coalesced, element k at (k / V) % LPR * V + ... as the
+    // lanes read it: vector c*LPR+lg holds k = (c*LPR+lg)*V .. +V-1
+    for (int k = tid; k < EPL * LPR; k += WAVES * 64)
+      s_f[k] = beta_at<F>(a.beta, k, K) - (k < K ? eps : (F)0);
+    __syncthreads();
 #pragma unroll
-  for (int c = 0; c < CH; ++c)
+    for (int c = 0; c < CH; ++c) {
+      F f[V];
+      load_vec<F>(s_f + (c * LPR + lg) * V, f);
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const int k = (c * LPR + lg) * V + v;
-      const F bk = k < K ? (F)a.beta[k] : (F)0;
-      f1[c * V + v] = bk - eps;  // link factor; the non-link factor is -f1
-      acc[c * V + v] = (F)0;
+      for (int v = 0; v < V; ++v) {
+        const int j = c * V + v;
+        q[j] = pii[j] * f[v];
+        Sp += pii[j];
+        acc[j] = (F)0;
+      }
     }
+    if (w == 0 && g == 0) {
+#pragma unroll
+      for (int c = 0; c < CH; ++c) store_vec<F>(s_pii + (c * LPR + lg) * V, pii + c * V);
+    }
+  }
+  const F S = group_sum<LPR>(Sp);
   const F e1 = eps, e0 = (F)1 - eps;
+  const F zs1 = e1 * S, zs0 = e0 * S;
+  F C = (F)0;
   // Otherwise the node's neighbour ids (bit 31 = link) are staged in LDS,
   // chunk by chunk.  The U ids of an iteration are read before any of its
   // rows is requested, so that the U rows' loads are all in flight together.
@@ -993,27 +1038,20 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        mask_row<F, LPR, CH>(K, lg, row[u]);
+        // q is zero past the row, so the loads past it need no mask here;
+        // acc past the row is never used
         const bool ok = (b0 + u * STRIDE < cn) && ((nbv[u] & 0x7fffffff) != node);
         F p = (F)0;
-        if (nbv[u] < 0) {  // link: f = beta - eps, e = eps
 #pragma unroll
-          for (int j = 0; j < EPL; ++j) {
-            row[u][j] = fma(row[u][j], f1[j], e1);  // x_bk
-            p = fma(pii[j], row[u][j], p);
-          }
-        } else {  // non-link: f = eps - beta, e = 1 - eps
-#pragma unroll
-          for (int j = 0; j < EPL; ++j) {
-            row[u][j] = fma(-row[u][j], f1[j], e0);
-            p = fma(pii[j], row[u][j], p);
-          }
-        }
+        for (int j = 0; j < EPL; ++j) p = fma(q[j], row[u][j], p);
         p = group_sum<LPR>(p);
-        const F wz = recip(p);
+        const bool link = nbv[u] < 0;
+        const F wz = recip(link ? p + zs1 : zs0 - p);
         if (ok) {
+          const F sw = link ? wz : -wz;
 #pragma unroll
-          for (int j = 0; j < EPL; ++j) acc[j] += fma(row[u][j], wz, (F)-1);
+          for (int j = 0; j < EPL; ++j) acc[j] = fma(row[u][j], sw, acc[j]);
+          C += fma(link ? e1 : e0, wz, (F)-1);
         }
       }
     }
@@ -1021,23 +1059,125 @@ __global__ __launch_bounds__(WAVES * 64) void k_update_phi(PhiArgs a) {
   if (RPW > 1) {
 #pragma unroll
     for (int j = 0; j < EPL; ++j) acc[j] = cross_group_sum<LPR>(acc[j]);
+    C = cross_group_sum<LPR>(C);
   }
-  if (WAVES > 1) {
-    if (w > 0 && g == 0) {
+  if (!reg_ids) __syncthreads();  // s_nb's last reads before s_acc is written
+  F* stg = (F*)a.staging + (int64_t)slot * a.ld;
+  if constexpr (LPR == 64 && WAVES > 1) {
+    // The phi step shared by the wavefronts: wavefront w takes the row's
+    // vector chunks c = w, w + WAVES, ... (its lanes' registers then hold a
+    // fraction of the row, not all of it).  The sums over wavefronts are
+    // added in wavefront order, the renormalisation sum per wavefront and
+    // then over wavefronts in order.
 #pragma unroll
-      for (int j = 0; j < EPL; ++j) s_acc[((w - 1) * EPL + j) * LPR + lg] = acc[j];
-    }
+    for (int j = 0; j < EPL; ++j) s_acc[(w * (EPL + 1) + j) * 64 + lane] = acc[j];
+    s_acc[(w * (EPL + 1) + EPL) * 64 + lane] = C;
     __syncthreads();
-    if (w != 0) return;
+    F Ct = (F)0;
 #pragma unroll
-    for (int q = 1; q < WAVES; ++q)
+    for (int q2 = 0; q2 < WAVES; ++q2) Ct += s_acc[(q2 * (EPL + 1) + EPL) * 64 + lane];
+    constexpr int CPW = (CH + WAVES - 1) / WAVES;  // chunks per wavefront
+    const F eps_tf = (F)eps_t;
+    const F half_eps = eps_tf / (F)2;
+    const F alpha = (F)a.hp.alpha;
+    const F Nn = (F)a.hp.Nn;
+    const F inv_phi_sum = recip(phi_sum);
+    const F* noise = (const F*)a.noise;
+    F phin[CPW * V];
+    double ssum = 0.0;
 #pragma unroll
-      for (int j = 0; j < EPL; ++j) acc[j] += s_acc[((q - 1) * EPL + j) * LPR + lg];
+    for (int cc = 0; cc < CPW; ++cc) {
+      const int c = w + cc * WAVES;
+#pragma unroll
+      for (int v = 0; v < V; ++v) phin[cc * V + v] = (F)0;
+      if (c >= CH) continue;  // uniform over the wavefront
+      const int k0 = (c * 64 + lane) * V;
+      F f[V], pv[V];
+      load_vec<F>(s_f + k0, f);
+      load_vec<F>(s_pii + k0, pv);
+#pragma unroll
+      for (int v = 0; v < V; v += 2) {
+        const int k = k0 + v;
+        F xi0 = (F)0, xi1 = (F)0;
+        if (k < K) {
+          if (noise) {
+            xi0 = noise[(int64_t)slot * K + k];
+            if (k + 1 < K) xi1 = noise[(int64_t)slot * K + k + 1];
+          } else {
+            normal_pair(draw4(a.key, (uint32_t)(k >> 1), (uint32_t)node, (uint32_t)t, TAG_PHI_NOISE),
+                        &xi0, &xi1);
+          }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          F accv = (F)0;
+#pragma unroll
+          for (int q2 = 0; q2 < WAVES; ++q2)
+            accv += s_acc[(q2 * (EPL + 1) + c * V + v + h) * 64 + lane];
+          const F grad = fma(f[v + h], accv, Ct) * inv_phi_sum;
+          const F phik = pv[v + h] * phi_sum;
+          F val = fabs(phik + half_eps * (alpha - phik + Nn * grad) +
+                       sqrt(eps_tf * phik) * (h ? xi1 : xi0));
+          if (val < (F)1e-24) val = (F)1e-24;
+          if (k + h >= K) val = (F)0;
+          phin[cc * V + v + h] = val;
+          ssum += (double)val;
+        }
+      }
+    }
+    ssum = group_sum<64>(ssum);
+    if (lane == 0) s_sum[w] = ssum;
+    __syncthreads();
+    double stot = 0.0;
+#pragma unroll
+    for (int q2 = 0; q2 < WAVES; ++q2) stot += s_sum[q2];
+    const double inv_s = 1.0 / stot;
+#pragma unroll
+    for (int cc = 0; cc < CPW; ++cc) {
+      const int c = w + cc * WAVES;
+      if (c >= CH) continue;
+      const int k0 = (c * 64 + lane) * V;
+      F out[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) out[v] = (F)((double)phin[cc * V + v] * inv_s);
+      if (k0 + V <= K) {
+        store_vec_nt<F>(stg + k0, out);
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          if (k0 + v < K) stg[k0 + v] = out[v];
+      }
+    }
+    if (w == 0 && lane == 0) stg[K] = (F)stot;
+  } else {
+    if (WAVES > 1) {
+      if (w > 0 && g == 0) {
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) s_acc[((w - 1) * (EPL + 1) + j) * LPR + lg] = acc[j];
+        s_acc[((w - 1) * (EPL + 1) + EPL) * LPR + lg] = C;
+      }
+      __syncthreads();
+      if (w != 0) return;
+#pragma unroll
+      for (int q2 = 1; q2 < WAVES; ++q2) {
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) acc[j] += s_acc[((q2 - 1) * (EPL + 1) + j) * LPR + lg];
+        C += s_acc[((q2 - 1) * (EPL + 1) + EPL) * LPR + lg];
+      }
+    }
+    if (g != 0) return;
+    // grads_k = f_k acc_k + C, and the node's pi row, for the phi step
+    F pii[EPL];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      F f[V];
+      load_vec<F>(s_f + (c * LPR + lg) * V, f);
+      load_vec<F>(s_pii + (c * LPR + lg) * V, pii + c * V);
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[c * V + v] = fma(f[v], acc[c * V + v], C);
+    }
+    phi_epilogue<F, LPR, CH>(a, slot, node, phi_sum, pii, acc, lg, stg, t, eps_t);
   }
-  if (g != 0) return;
-
-  phi_epilogue<F, LPR, CH>(a, slot, node, phi_sum, pii, acc, lg,
-                           (F*)a.staging + (int64_t)slot * a.ld, t, eps_t);
 }
 
 // Publish staged rows: pi[nodes[slot]] = staging[slot] (K+1 elements).
@@ -1108,7 +1248,7 @@ __device__ __forceinline__ void beta_body(const BetaArgs& a, const int bx) {
     for (int v = 0; v < V; ++v) {
       const int k = (c * LPR + lg) * V + v;
       const int j = c * V + v;
-      bt[j] = k < K ? (F)a.beta[k] : (F)0;
+      bt[j] = beta_at<F>(a.beta, k, K);
       g0[j] = 0.0;
       g1[j] = 0.0;
     }
@@ -1334,7 +1474,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_ppx(PpxArgs a) {
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       const int k = (c * LPR + lg) * V + v;
-      bt[c * V + v] = k < K ? (F)a.beta[k] : (F)0;
+      bt[c * V + v] = beta_at<F>(a.beta, k, K);
     }
   double ll1 = 0.0, ll0 = 0.0, n1 = 0.0, n0 = 0.0;
   const double cnt = a.avg_count;
@@ -2687,7 +2827,7 @@ __device__ __forceinline__ void phi_batch_body(const PhiArgs& a, int item, int n
       const int k = koff[c] + v;
       const bool in = koff[c] < K && k < K;
       pii[c * V + v] = in ? pii[c * V + v] : (F)0;
-      f1[c * V + v] = (in ? (F)a.beta[k] : (F)0) - eps;
+      f1[c * V + v] = (in ? beta_at<F>(a.beta, k, K) : (F)0) - eps;
       acc[c * V + v] = (F)0;
     }
   const F e1 = eps, e0 = (F)1 - eps;
