@@ -747,11 +747,15 @@ __device__ __forceinline__ void neighbors_body(const NeighborArgs& a, int32_t sl
   const int rslot = rslot0 + by;  // ring slot (iteration)
   const int slot = slot_offset + bx * WAVES + w;  // node slot
   const IterMeta& mt = a.ring.meta[rslot];
+  // the node id is read together with the slot's count (its index clamped
+  // into the slot), not after the count test: one dependent load less
+  const int32_t node_any =
+      a.ring.nodes[(int64_t)rslot * a.ring.node_cap + min(slot, (int32_t)a.ring.node_cap - 1)];
   const int count = mt.node_count;
   if (slot >= count || slot >= slot_end) return;
   const uint32_t t32 = (uint32_t)mt.step;
   const uint32_t N = (uint32_t)a.g.N;
-  const int32_t node = a.ring.nodes[(int64_t)rslot * a.ring.node_cap + slot];
+  const int32_t node = node_any;
   // LDS per wavefront: the de-dup table, then room for the node's row
   int2* tab = a.use_lds ? s_tab + (size_t)w * (a.tcap + kNbrRowLds / 2)
                         : a.gtable + (size_t)(slot - slot_offset) * a.tcap;
