@@ -350,8 +350,8 @@ int ammsb_get_tuning(ammsb_ctx *ctx, const char *name, int64_t *value);
 int ammsb_probe_read(ammsb_ctx *ctx, int32_t reps, double *seconds, int64_t *bytes);
 /* Bandwidth probe (no counterpart in the reference): the random row gather
  * of update_phi for iteration step_count -- its minibatch nodes' own rows
- * and n+1 neighbour rows, read with update_phi's default launch shape --
- * without the arithmetic and the row writes; `reps` launches, *seconds =
+ * and n+1 neighbour rows, read with the context's update_phi launch shape
+ * when it is 8 x 2, else 4 x 1 -- without the arithmetic and the row writes; `reps` launches, *seconds =
  * average time of one, *bytes = rows and neighbour ids read.  The state is
  * not changed; the sample-ahead ring is refilled before the next step. */
 int ammsb_probe_gather(ammsb_ctx *ctx, int32_t reps, double *seconds, int64_t *bytes);

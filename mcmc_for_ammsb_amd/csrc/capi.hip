@@ -1499,10 +1499,15 @@ int ammsb_probe_gather(ammsb_ctx* c, int32_t reps, double* seconds, int64_t* byt
     return AMMSB_ERR_HIP;
   }
   const PhiArgs pa = phi_args(c);
-  HIP_TRY(launch_gather_probe(pa, c->prec, (int32_t)c->node_cap, c->d_ppx_partials, c->stream));
+  // update_phi's own launch shape when the probe has it, else 4 x 1
+  const bool s82 = c->phi_waves == 8 && c->phi_unroll == 2;
+  const int pw = s82 ? 8 : 4, pu = s82 ? 2 : 1;
+  HIP_TRY(launch_gather_probe(pa, c->prec, (int32_t)c->node_cap, c->d_ppx_partials, c->stream, pw,
+                              pu));
   HIP_TRY(hipEventRecord(a, c->stream));
   for (int32_t r = 0; r < reps; ++r)
-    HIP_TRY(launch_gather_probe(pa, c->prec, (int32_t)c->node_cap, c->d_ppx_partials, c->stream));
+    HIP_TRY(launch_gather_probe(pa, c->prec, (int32_t)c->node_cap, c->d_ppx_partials, c->stream,
+                                pw, pu));
   HIP_TRY(hipEventRecord(b, c->stream));
   HIP_TRY(hipEventSynchronize(b));
   float ms = 0.f;

@@ -267,10 +267,11 @@ hipError_t launch_label_nbrs(DevGraph g, const int32_t* nodes, int32_t n_nodes, 
 namespace ammsb {
 // Contiguous read of `bytes` (a multiple of 16) starting at p.
 hipError_t launch_read_probe(const void* p, int64_t bytes, double* sink, hipStream_t s);
-// update_phi's row gather of the current ring iteration (default launch
-// shape) without the arithmetic or the row write.
+// update_phi's row gather of the current ring iteration without the
+// arithmetic or the row write, in launch shape (waves, unroll): (4, 1), or
+// (8, 2) where update_phi has it.
 hipError_t launch_gather_probe(const PhiArgs& a, int precision, int32_t grid_nodes, double* sink,
-                               hipStream_t s);
+                               hipStream_t s, int waves, int unroll);
 }  // namespace ammsb
 
 namespace ammsb {

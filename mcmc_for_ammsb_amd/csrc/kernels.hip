@@ -1906,12 +1906,12 @@ hipError_t launch_read_probe(const void* p, int64_t bytes, double* sink, hipStre
 
 namespace {
 // Bandwidth probe of k_update_phi's row gather: the same workgroup per
-// minibatch node, wavefront split, row groups, neighbour-id registers and
-// streaming row loads as k_update_phi (default shape), without the
-// arithmetic and without the staged row write.  The rows are summed into a
-// value stored only if it equals an impossible value, so no load can be
-// removed.
-template <typename F, int LPR, int CH, int WAVES>
+// minibatch node, wavefront split, row groups, rows in flight, neighbour-id
+// registers and streaming row loads as k_update_phi (the context's launch
+// shape), without the arithmetic and without the staged row write.  The rows
+// are summed into a value stored only if it equals an impossible value, so
+// no load can be removed.
+template <typename F, int LPR, int CH, int WAVES, int U>
 __global__ __launch_bounds__(WAVES * 64) void k_gather_probe(PhiArgs a, double* sink) {
   constexpr int V = VT<F>::V;
   constexpr int EPL = V * CH;
@@ -1939,41 +1939,60 @@ __global__ __launch_bounds__(WAVES * 64) void k_gather_probe(PhiArgs a, double* 
   F acc = (F)0;
 #pragma unroll
   for (int j = 0; j < EPL; ++j) acc += row[j];
-  for (int it = 0; w * RPW + it * STRIDE < a.nnb; ++it) {
-    const int b = w * RPW + g + it * STRIDE;
-    int32_t v;
-    if (reg_ids) {
-      if constexpr (RPW == 1)
-        v = __builtin_amdgcn_readlane(ids, min(it, 63));
-      else
-        v = __shfl(ids, (it * RPW + g) & 63, 64);
-    } else {
-      v = b < a.nnb ? nb_src[b] : node;
-    }
-    v = b < a.nnb ? v : node;
-    if (LPR == 64) v = __builtin_amdgcn_readfirstlane(v);
-    issue_row<F, LPR, CH, true>(pi + (int64_t)(v & 0x7fffffff) * a.ld, K, lg, row);
+  for (int it = 0; w * RPW + it * STRIDE < a.nnb; it += U) {
+    F rows[U][EPL];
 #pragma unroll
-    for (int j = 0; j < EPL; ++j) acc += row[j];
+    for (int u = 0; u < U; ++u) {
+      const int b = w * RPW + g + (it + u) * STRIDE;
+      int32_t v;
+      if (reg_ids) {
+        if constexpr (RPW == 1)
+          v = __builtin_amdgcn_readlane(ids, min(it + u, 63));
+        else
+          v = __shfl(ids, ((it + u) * RPW + g) & 63, 64);
+      } else {
+        v = b < a.nnb ? nb_src[b] : node;
+      }
+      v = b < a.nnb ? v : node;
+      if (LPR == 64) v = __builtin_amdgcn_readfirstlane(v);
+      issue_row<F, LPR, CH, true>(pi + (int64_t)(v & 0x7fffffff) * a.ld, K, lg, rows[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int j = 0; j < EPL; ++j) acc += rows[u][j];
   }
   if (acc == (F)-1.2345e30) sink[0] = (double)acc;
 }
 }  // namespace
 
+namespace {
+template <typename F, int L, int C>
+hipError_t probe_launch(const PhiArgs& a, int grid, double* sink, hipStream_t s, int waves,
+                        int unroll) {
+  if (waves == 4 && unroll == 1) {
+    hipLaunchKernelGGL((k_gather_probe<F, L, C, 4, 1>), dim3(grid), dim3(256), 0, s, a, sink);
+    return hipGetLastError();
+  }
+  if constexpr (L == 64 && (C == 1 || C == 2 || C == 4 || C == 8)) {
+    if (waves == 8 && unroll == 2) {
+      hipLaunchKernelGGL((k_gather_probe<F, L, C, 8, 2>), dim3(grid), dim3(512), 0, s, a, sink);
+      return hipGetLastError();
+    }
+  }
+  return hipErrorInvalidValue;
+}
+}  // namespace
+
 hipError_t launch_gather_probe(const PhiArgs& a, int precision, int32_t grid_nodes, double* sink,
-                               hipStream_t s) {
+                               hipStream_t s, int waves, int unroll) {
   int lpr, ch;
   if (!row_config(a.K, precision, &lpr, &ch)) return hipErrorInvalidValue;
   if (grid_nodes <= 0) return hipSuccess;
-#define X(L, C)                                                                              \
-  if (lpr == L && ch == C) {                                                                 \
-    if (precision == 0)                                                                      \
-      hipLaunchKernelGGL((k_gather_probe<double, L, C, 4>), dim3(grid_nodes), dim3(256), 0, s, \
-                         a, sink);                                                           \
-    else                                                                                     \
-      hipLaunchKernelGGL((k_gather_probe<float, L, C, 4>), dim3(grid_nodes), dim3(256), 0, s,  \
-                         a, sink);                                                           \
-    return hipGetLastError();                                                                \
+#define X(L, C)                                                                       \
+  if (lpr == L && ch == C) {                                                          \
+    if (precision == 0) return probe_launch<double, L, C>(a, grid_nodes, sink, s, waves, unroll); \
+    return probe_launch<float, L, C>(a, grid_nodes, sink, s, waves, unroll);           \
   }
   AMMSB_ROW_CONFIGS(X)
 #undef X
