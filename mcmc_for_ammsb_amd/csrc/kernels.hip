@@ -89,23 +89,35 @@ template <>
 struct NTV<float> {
   typedef st_f4 type;
 };
-template <typename F>
-__device__ __forceinline__ void store_vec_nt(F* p, const F* in);
-template <>
-__device__ __forceinline__ void store_vec_nt<double>(double* p, const double* in) {
-  st_d2 v;
-  v.x = in[0];
-  v.y = in[1];
-  __builtin_nontemporal_store(v, reinterpret_cast<st_d2*>(p));
+
+// Accesses through pointers that address global memory.  A pointer read
+// from memory (the batched kernels' per-chain arguments) is generic to the
+// compiler, which then issues flat instructions: they take 64-bit addresses
+// and also count against the LDS/scalar counter, so a wait for one row
+// becomes a wait for every load in flight.  Only for pointers into device
+// memory (never LDS).
+template <typename T>
+using gptr = T __attribute__((address_space(1)))*;
+template <typename T>
+__device__ __forceinline__ T ld_g(const T* p) {
+  return *(gptr<const T>)p;
 }
-template <>
-__device__ __forceinline__ void store_vec_nt<float>(float* p, const float* in) {
-  st_f4 v;
-  v.x = in[0];
-  v.y = in[1];
-  v.z = in[2];
-  v.w = in[3];
-  __builtin_nontemporal_store(v, reinterpret_cast<st_f4*>(p));
+template <typename F>
+__device__ __forceinline__ void load_vec_g(const F* p, F* out) {
+  typedef typename NTV<F>::type T;
+  const T v = *(gptr<const T>)p;
+#pragma unroll
+  for (int i = 0; i < VT<F>::V; ++i) out[i] = v[i];
+}
+
+// 16-byte non-temporal store (p in device memory, 16-byte aligned).
+template <typename F>
+__device__ __forceinline__ void store_vec_nt(F* p, const F* in) {
+  typedef typename NTV<F>::type T;
+  T v;
+#pragma unroll
+  for (int i = 0; i < VT<F>::V; ++i) v[i] = in[i];
+  __builtin_nontemporal_store(v, (gptr<T>)p);
 }
 
 // beta[k] in Float, 0 past the row (K >= 1).  The load is unconditional,
@@ -113,7 +125,7 @@ __device__ __forceinline__ void store_vec_nt<float>(float* p, const float* in) {
 // with its own wait, and a row's worth of them run one after another.
 template <typename F>
 __device__ __forceinline__ F beta_at(const double* beta, int k, int K) {
-  const double b = beta[k < K ? k : K - 1];
+  const double b = ld_g(beta + (k < K ? k : K - 1));
   return k < K ? (F)b : (F)0;
 }
 
@@ -124,13 +136,13 @@ template <typename F>
 __device__ __forceinline__ void load_vec_nt(const F* p, F* out);
 template <>
 __device__ __forceinline__ void load_vec_nt<double>(const double* p, double* out) {
-  const nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
+  const nt_d2 v = __builtin_nontemporal_load((gptr<const nt_d2>)p);
   out[0] = v.x;
   out[1] = v.y;
 }
 template <>
 __device__ __forceinline__ void load_vec_nt<float>(const float* p, float* out) {
-  const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+  const nt_f4 v = __builtin_nontemporal_load((gptr<const nt_f4>)p);
   out[0] = v.x;
   out[1] = v.y;
   out[2] = v.z;
@@ -150,7 +162,7 @@ __device__ __forceinline__ void issue_row(const F* row, int K, int lg, F* out) {
     if constexpr (NT)
       load_vec_nt<F>(row + (k0 < K ? k0 : 0), out + c * V);
     else
-      load_vec<F>(row + (k0 < K ? k0 : 0), out + c * V);
+      load_vec_g<F>(row + (k0 < K ? k0 : 0), out + c * V);
   }
 }
 // Zero the elements k >= K (selects, no branches around memory operations).
@@ -1272,9 +1284,9 @@ __device__ __forceinline__ void beta_body(const BetaArgs& a, const int bx) {
       const int e = e0 + u * gstride;
       val[u] = e < count;
       const int ee = val[u] ? e : e0;
-      ea[u] = edges[2 * ee];
-      eb[u] = edges[2 * ee + 1];
-      yv[u] = ys[ee] ? 1 : 0;
+      ea[u] = ld_g(edges + 2 * ee);
+      eb[u] = ld_g(edges + 2 * ee + 1);
+      yv[u] = ld_g(ys + ee) ? 1 : 0;
     }
     F ra[UE][EPL], rb[UE][EPL];
 #pragma unroll
@@ -1391,8 +1403,8 @@ __device__ __forceinline__ void theta_body(const ThetaArgs& a, const int bx, con
   if (k < K) {
 #pragma unroll 8
     for (int b = rg; b < a.blocks; b += kThetaRows) {
-      G0 += a.partials[(int64_t)b * 2 * K + k];
-      G1 += a.partials[(int64_t)b * 2 * K + K + k];
+      G0 += ld_g(a.partials + (int64_t)b * 2 * K + k);
+      G1 += ld_g(a.partials + (int64_t)b * 2 * K + K + k);
     }
   }
   s_g[rg][0][kl] = G0;
@@ -2803,12 +2815,14 @@ __device__ __forceinline__ int chain_of(const int32_t* off, int n, int item) {
 // G = kBatchWaves*64/lpr row groups are split evenly, gpn = G/npb groups
 // per node: gpn < 64/lpr packs several nodes into one wavefront, gpn >
 // 64/lpr spreads a node over several wavefronts.  Each group streams whole
-// neighbour rows (U in flight), reduces Z_b over its lanes and accumulates
-// x_bk/Z_b - 1; the node's groups are combined by a butterfly inside the
-// wavefront, then across its wavefronts through LDS.  The node's leader
-// group applies the phi step (noise, renormalisation) and stages the new
-// row with its phi_sum; k_scatter_batch publishes it once every gather of
-// the iteration is done.
+// neighbour rows (U in flight), reduces P_b over its lanes and accumulates
+// the regrouped gradient sums of k_update_phi (acc_k = sum_b s_b pi_bk /
+// Z_b and C = sum_b (e_b / Z_b - 1), so grads_k = f_k acc_k + C); the
+// node's groups are combined by a butterfly inside the wavefront, then
+// across its wavefronts through LDS.  The node's leader group applies the
+// phi step (noise, renormalisation) and stages the new row with its
+// phi_sum; k_scatter_batch publishes it once every gather of the iteration
+// is done.
 template <typename F, int CH, int U>
 __device__ __forceinline__ void phi_batch_body(const PhiArgs& a, int item, int npb, int lpr,
                                                int rs, int count, F* s_red) {
@@ -2825,35 +2839,43 @@ __device__ __forceinline__ void phi_batch_body(const PhiArgs& a, int item, int n
   const int slot = first + j;
   const bool active = slot < count;
   const int32_t* nodes = a.ring.nodes + (int64_t)rs * a.ring.node_cap;
-  const int32_t node = nodes[active ? slot : first];
+  const int32_t node = ld_g(nodes + (active ? slot : first));
   const int32_t* nb = a.ring.nbrs + ((int64_t)rs * a.ring.node_cap + (active ? slot : first)) * a.nnb;
   const F* pi = (const F*)a.pi;
   const int K = a.K;
   const F eps = (F)a.hp.epsilon;
-  F pii[EPL], f1[EPL], acc[EPL];
+  F q[EPL], acc[EPL];
   int koff[CH];  // first element of each of the lane's vectors (K: past the row)
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     const int k0 = (c * lpr + lg) * V;
     koff[c] = k0 < K ? k0 : K;
   }
+  // The node's own row: q_k = pi_ak f_k and S here; the leader reads it
+  // again for the phi step (fewer registers in the row loop).
   const F* prow = pi + (int64_t)node * a.ld;
 #pragma unroll
-  for (int c = 0; c < CH; ++c) load_vec<F>(prow + (koff[c] < K ? koff[c] : 0), pii + c * V);
-  const F phi_sum = prow[K];
-  const int64_t t = a.st->step;
+  for (int c = 0; c < CH; ++c) load_vec_g<F>(prow + (koff[c] < K ? koff[c] : 0), q + c * V);
+  const F phi_sum = ld_g(prow + K);
+  const int64_t t = ld_g(&a.st->step);
   const double eps_t = phi_step_size(a);
+  F Sp = (F)0;
 #pragma unroll
   for (int c = 0; c < CH; ++c)
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       const int k = koff[c] + v;
       const bool in = koff[c] < K && k < K;
-      pii[c * V + v] = in ? pii[c * V + v] : (F)0;
-      f1[c * V + v] = (in ? beta_at<F>(a.beta, k, K) : (F)0) - eps;
-      acc[c * V + v] = (F)0;
+      const int jj = c * V + v;
+      const F pk = in ? q[jj] : (F)0;
+      q[jj] = pk * (beta_at<F>(a.beta, k, K) - eps);
+      Sp += pk;
+      acc[jj] = (F)0;
     }
+  const F S = group_sum_rt(Sp, lpr);
   const F e1 = eps, e0 = (F)1 - eps;
+  const F zs1 = e1 * S, zs0 = e0 * S;
+  F C = (F)0;
   const int nnb = a.nnb;
   // neighbour ids one iteration ahead, so that an iteration's row loads do
   // not wait for its id loads
@@ -2861,7 +2883,7 @@ __device__ __forceinline__ void phi_batch_body(const PhiArgs& a, int item, int n
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int b = r + u * gpn;
-    nxt[u] = b < nnb ? nb[b] : node;  // past the end: own row, not counted
+    nxt[u] = b < nnb ? ld_g(nb + b) : node;  // past the end: own row, not counted
   }
   for (int b0 = r; b0 < nnb; b0 += U * gpn) {
     int32_t nbv[U];
@@ -2871,62 +2893,70 @@ __device__ __forceinline__ void phi_batch_body(const PhiArgs& a, int item, int n
       nbv[u] = nxt[u];
       const F* rr = pi + (int64_t)(nbv[u] & 0x7fffffff) * a.ld;
 #pragma unroll
-      for (int c = 0; c < CH; ++c) load_vec<F>(rr + (koff[c] < K ? koff[c] : 0), row[u] + c * V);
+      for (int c = 0; c < CH; ++c) load_vec_g<F>(rr + (koff[c] < K ? koff[c] : 0), row[u] + c * V);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int b = b0 + (U + u) * gpn;
-      nxt[u] = b < nnb ? nb[b] : node;
+      nxt[u] = b < nnb ? ld_g(nb + b) : node;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
+      // q is zero past the row, so p needs no mask; acc past the row is
+      // never used
       const bool ok = (b0 + u * gpn < nnb) && ((nbv[u] & 0x7fffffff) != node);
       F p = (F)0;
-      // pii is zero past the row, so p needs no mask; x_bk past the row is
-      // never stored
-      if (nbv[u] < 0) {
 #pragma unroll
-        for (int jj = 0; jj < EPL; ++jj) {
-          row[u][jj] = fma(row[u][jj], f1[jj], e1);
-          p = fma(pii[jj], row[u][jj], p);
-        }
-      } else {
-#pragma unroll
-        for (int jj = 0; jj < EPL; ++jj) {
-          row[u][jj] = fma(-row[u][jj], f1[jj], e0);
-          p = fma(pii[jj], row[u][jj], p);
-        }
-      }
+      for (int jj = 0; jj < EPL; ++jj) p = fma(q[jj], row[u][jj], p);
       p = group_sum_rt(p, lpr);
-      const F wz = recip(p);
+      const bool link = nbv[u] < 0;
+      const F wz = recip(link ? p + zs1 : zs0 - p);
       if (ok) {
+        const F sw = link ? wz : -wz;
 #pragma unroll
-        for (int jj = 0; jj < EPL; ++jj) acc[jj] += fma(row[u][jj], wz, (F)-1);
+        for (int jj = 0; jj < EPL; ++jj) acc[jj] = fma(row[u][jj], sw, acc[jj]);
+        C += fma(link ? e1 : e0, wz, (F)-1);
       }
     }
   }
   // combine the node's groups inside the wavefront: lanes of a node are
   // contiguous, min(gpn, rpw)*lpr of them
   const int span = (gpn < rpw ? gpn : rpw) * lpr;
-  for (int off = lpr; off < span; off <<= 1)
+  for (int off = lpr; off < span; off <<= 1) {
 #pragma unroll
     for (int jj = 0; jj < EPL; ++jj) acc[jj] += __shfl_xor(acc[jj], off, 64);
+    C += __shfl_xor(C, off, 64);
+  }
   bool leader = r == 0;
   if (gpn > rpw) {  // block-uniform: the node spans wpn wavefronts
     const int wpn = gpn / rpw;
     if ((w % wpn) != 0 && g == 0) {
 #pragma unroll
-      for (int jj = 0; jj < EPL; ++jj) s_red[(w * EPL + jj) * 64 + lg] = acc[jj];
+      for (int jj = 0; jj < EPL; ++jj) s_red[(w * (EPL + 1) + jj) * 64 + lg] = acc[jj];
+      s_red[(w * (EPL + 1) + EPL) * 64 + lg] = C;
     }
     __syncthreads();
     if ((w % wpn) == 0 && g == 0) {
-      for (int q = 1; q < wpn; ++q)
+      for (int q2 = 1; q2 < wpn; ++q2) {
 #pragma unroll
-        for (int jj = 0; jj < EPL; ++jj) acc[jj] += s_red[((w + q) * EPL + jj) * 64 + lg];
+        for (int jj = 0; jj < EPL; ++jj) acc[jj] += s_red[((w + q2) * (EPL + 1) + jj) * 64 + lg];
+        C += s_red[((w + q2) * (EPL + 1) + EPL) * 64 + lg];
+      }
     }
     leader = (w % wpn) == 0 && g == 0;
   }
   if (!leader || !active) return;
+  // grads_k = f_k acc_k + C, and the node's pi row (past the row: unused)
+  F pii[EPL];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    load_vec_g<F>(prow + (koff[c] < K ? koff[c] : 0), pii + c * V);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int jj = c * V + v;
+      acc[jj] = fma(beta_at<F>(a.beta, koff[c] + v, K) - eps, acc[jj], C);
+    }
+  }
   phi_epilogue<F, 0, CH>(a, slot, node, phi_sum, pii, acc, lg,
                          (F*)a.staging + (int64_t)slot * a.ld, t, eps_t, lpr);
 }
@@ -2944,7 +2974,7 @@ __global__ __launch_bounds__(kBatchWaves * 64)
 __attribute__((amdgpu_waves_per_eu(sizeof(F) == 4 ? 5 : 3))) void k_update_phi_batch(
     const PhiArgs* __restrict__ args, const BatchPhiShape* __restrict__ shapes, int n,
     BatchIndex ix) {
-  __shared__ F s_red[kBatchWaves * 8 * 64];
+  __shared__ F s_red[kBatchWaves * (8 + 1) * 64];
   __shared__ int s_item;
   extern __shared__ int4 s_info[];  // batch_layout_lds(n) bytes
   int32_t* s_off = reinterpret_cast<int32_t*>(s_info + n);
@@ -3046,10 +3076,10 @@ __global__ __launch_bounds__(256) void k_scatter_batch(const PhiArgs* __restrict
     const PhiArgs& a = args[c];
     const int slot = (it - __builtin_amdgcn_readfirstlane(s_off[p])) * (64 / lpr) + lane / lpr;
     if (slot >= count) continue;
-    const int32_t node = a.ring.nodes[(int64_t)rs * a.ring.node_cap + slot];
+    const int32_t node = ld_g(a.ring.nodes + (int64_t)rs * a.ring.node_cap + slot);
     const int nv = (a.K + 1 + V - 1) / V;
-    const T* src = reinterpret_cast<const T*>((const F*)a.staging + (int64_t)slot * a.ld);
-    T* dst = reinterpret_cast<T*>((F*)a.pi + (int64_t)node * a.ld);
+    gptr<const T> src = (gptr<const T>)((const F*)a.staging + (int64_t)slot * a.ld);
+    gptr<T> dst = (gptr<T>)((F*)a.pi + (int64_t)node * a.ld);
     for (int i = lane % lpr; i < nv; i += lpr) __builtin_nontemporal_store(src[i], dst + i);
   }
 }
