@@ -325,10 +325,10 @@ int ammsb_get_totals(ammsb_ctx *ctx, int64_t *nodes_updated,
  *                wider than 2 KB): 2, 4, 8
  *   "ll_fwd_prefetch", "ll_bwd_prefetch"  1: the next pairs' rows are
  *                loaded before the exchange
- *   "ll_fwd_rows", "ll_bwd_rows"  1: the forward / adjoint pass of rows
- *                wider than 2 KB streams whole rows per wavefront against
- *                the source row held in LDS; 0: the K-chunked kernels above
- *                (defaults: forward 1; adjoint 1 for FP64, 0 for FP32)
+ *   "ll_fwd_rows", "ll_bwd_rows"  1 (default): the forward / adjoint pass
+ *                of rows wider than 2 KB streams whole rows per wavefront
+ *                against the source row held in registers; 0: the
+ *                K-chunked kernels above
  * Set phi_waves first; the (waves, unroll) pair is validated when
  * phi_unroll is set.  Results do not depend on the shape beyond
  * summation order (the log-likelihood shapes not even on that).  Defaults:

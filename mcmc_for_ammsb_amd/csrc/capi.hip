@@ -121,7 +121,7 @@ struct ammsb_ctx {
   // K-chunked log-likelihood launch shapes [0] forward, [1] adjoint
   int32_t ll_pairs[2] = {0, 0}, ll_prefetch[2] = {0, 0};
   int32_t ll_fwd_rows = 1;  // forward of wide rows: whole rows per wavefront (else K-chunked)
-  int32_t ll_bwd_rows = 1;  // adjoint of wide rows: the same (default FP64 only)
+  int32_t ll_bwd_rows = 1;  // adjoint of wide rows: the same
   int32_t cus = 256;        // compute units of the device
   double* d_ll_scratch = nullptr;   // differentiation: per-source values / dbeta partials
   size_t ll_scratch_bytes = 0;
@@ -595,10 +595,9 @@ int create_impl(const ammsb_options* opts, const ammsb_graph* g, const ammsb_spl
   c->ll_prefetch[0] = 0;
   c->ll_pairs[1] = c->prec == AMMSB_FP64 ? 4 : 2;
   c->ll_prefetch[1] = c->prec == AMMSB_FP64 ? 0 : 1;
-  // whole-row adjoint: faster for FP64 (N=1M, K=2048, 2.5M pairs: 8.6 -> 7.4
-  // ms), slower for FP32 (4.16 -> 4.36 ms: its LDS traffic per byte of row
-  // doubles)
-  c->ll_bwd_rows = c->prec == AMMSB_FP64 ? 1 : 0;
+  // whole-row adjoint in both precisions (N=1M, K=2048, 2.5M pairs, against
+  // the K-chunked one: FP64 6.9 against 8.6 ms, FP32 3.5 against 4.2 ms)
+  c->ll_bwd_rows = 1;
   {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, o.device) ==
@@ -1537,20 +1536,13 @@ int loglik_common(ammsb_ctx* c, int64_t M, const int32_t* nodes, const void* phi
   HIP_TRY(hipSetDevice(c->opt.device));
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   if (stream) HIP_TRY(hipStreamSynchronize(c->stream));  // resident state settled
-  // the whole-row adjoint holds (2 + 4) row-sized arrays in LDS per
-  // workgroup (FP64 K=2048: 96 KB, one workgroup per CU; FP32: 48 KB, three)
-  // and writes a beta partial row per wavefront: as many workgroups as are
-  // resident at once
+  // the whole-row adjoint writes one beta partial row per workgroup: as
+  // many workgroups as are resident at once (FP64 K=2048: one per CU)
   const bool rows_bwd = bwd && c->ll_bwd_rows && loglik_rows_chunks(c->K, c->prec) > 0;
   int64_t max_blocks = 1024;
-  if (rows_bwd) {
-    const int64_t per_cu = std::max<int64_t>(
-        1, std::min<int64_t>(4, (int64_t)kLdsPerCu /
-                                    (int64_t)(loglik_rows_bwd_lds(c->K, c->prec) + 256)));
-    max_blocks = per_cu * c->cus;
-  }
+  if (rows_bwd) max_blocks = (int64_t)loglik_rows_blocks_per_cu(c->K, c->prec) * c->cus;
   const int32_t blocks = (int32_t)std::max<int64_t>(1, std::min<int64_t>(M, max_blocks));
-  const int64_t prows = rows_bwd ? (int64_t)blocks * kLlRowWaves : blocks;
+  const int64_t prows = blocks;
   // scratch: per-source values (fwd) or [prows][K] dbeta partials (bwd), plus
   // one double for an absent total and the error word
   const int64_t body = bwd ? prows * c->K : M;

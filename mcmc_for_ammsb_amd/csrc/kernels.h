@@ -301,19 +301,21 @@ struct LogLikArgs {
 // pairs/prefetch: launch shape of the K-chunked kernel (pairs per LDS
 // exchange, next pairs' loads issued before the exchange); ignored when one
 // wavefront holds a row
-// rows: the forward pass of rows wider than 2 KB runs whole rows per
-// wavefront against the source row in LDS (k_loglik_rows) instead of the
-// K-chunked kernel
+// rows: rows wider than 2 KB run whole rows per wavefront against the
+// source row held in registers (k_loglik_rows) instead of the K-chunked
+// kernel
 hipError_t launch_loglik(const LogLikArgs& a, int precision, bool backward, hipStream_t s,
                          int pairs, int prefetch, bool rows);
 constexpr int kLlRowWaves = 4;  // wavefronts per workgroup of k_loglik_rows(_bwd)
 constexpr int64_t kLdsPerCu = 160 * 1024;  // LDS bytes per compute unit (gfx950)
 // 16-byte vectors per lane of a whole row (4, 8 or 16) for the whole-row
 // kernels; 0 where one wavefront's 2 KB holds the row or the row is too wide.
-// With rows, the adjoint writes blocks * kLlRowWaves beta partial rows.
+// With rows, the adjoint writes one beta partial row per workgroup.
 int loglik_rows_chunks(int32_t K, int precision);
-// Dynamic LDS bytes of one k_loglik_rows_bwd workgroup (0: not used for K)
-size_t loglik_rows_bwd_lds(int32_t K, int precision);
+// Dynamic LDS bytes of one k_loglik_rows workgroup (0: not used for K)
+size_t loglik_rows_lds(int32_t K, int precision, bool backward);
+// Resident workgroups per compute unit of the whole-row adjoint
+int loglik_rows_blocks_per_cu(int32_t K, int precision);
 bool loglik_kc_shape_supported(int pairs, int prefetch);
 // Wavefronts per source of the K-chunked log-likelihood kernel used for rows
 // wider than 2 KB, 0 where one wavefront per row is used.

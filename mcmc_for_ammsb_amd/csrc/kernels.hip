@@ -149,6 +149,24 @@ __device__ __forceinline__ void load_vec_nt<float>(const float* p, float* out) {
   out[3] = v.w;
 }
 
+// Whole rows read by one wavefront through a buffer resource: the row's
+// address in scalar registers (it is the same for every lane), each lane's
+// byte offsets in vector registers, computed once (4 bytes per vector
+// instead of a 64-bit address per vector and row).  bytes bounds the
+// offsets (the hardware returns 0 past it).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* row, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(row), (short)0, (int)bytes,
+                                           0x00020000);
+}
+// 16 bytes at byte offset off; NT: a non-temporal load.
+template <typename F, bool NT>
+__device__ __forceinline__ void load_vec_buf(__amdgpu_buffer_rsrc_t r, uint32_t off, F* out) {
+  typedef typename NTV<F>::type T;
+  const T v = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, NT ? 2 : 0));
+#pragma unroll
+  for (int i = 0; i < VT<F>::V; ++i) out[i] = v[i];
+}
+
 // Row of K elements into registers: vector vi = c*LPR + lg.  The loads are
 // unconditional so that every vector of a row (and of several rows) is in
 // flight before the first wait: a vector past the end of the row re-reads
@@ -2385,21 +2403,47 @@ __global__ __launch_bounds__(kLlMaxWaves * 64) void k_loglik_kc(LogLikArgs a) {
   }
 }
 
-// Forward log-likelihood for rows wider than one wavefront's 2 KB, whole
-// rows per wavefront: the source's normalised pi row and the link factors
-// live in LDS, each wavefront streams whole neighbour rows (NCH 16-byte
-// vectors per lane), so the wavefronts of a source never wait for one
-// another.  Pair p of a source goes to wavefront (p - p0) mod 4; each
-// wavefront's log-sum runs in pair order and the four are added in
-// wavefront order.
-template <typename F, int NCH>
+// Log-likelihood and its adjoint for rows wider than one wavefront's 2 KB,
+// whole rows per wavefront.  Pair p of a source goes to wavefront
+// (p - p0) mod 4, which holds the ids and labels of its next 64 pairs in
+// two registers (lane j: its j-th pair), so that requesting a row never
+// waits for a load; up to 8 KB rows it keeps two neighbour rows in flight
+// (the next pair's row is requested before the current one is reduced).
+// The source row is held in registers in the regrouped form of update_phi's
+// gradient:
+//   q_k = pi_sk f1_k,  S = sum_k pi_sk,  P_p = sum_k q_k pi_bk,
+//   Z_p = P_p + eps S (link) or (1 - eps) S - P_p (non-link),
+// one multiply-add per element and pair.  The adjoint accumulates per
+// wavefront and source
+//   A_k = sum_p s_p pi_bk / Z_p  (s_p = +1 link, -1 non-link),
+//   C = sum_p (e_p / Z_p - 1),
+// so that dL/dphi_sk = (f1_k A_k + C) / sum(phi_s) and the source's share
+// of dL/dbeta_k is pi_sk A_k (the sums of the x/Z - 1 and
+// (2y-1) pi_s pi_b / Z terms above, in another order).  At the end of a
+// source the wavefronts' A and C meet through two LDS buffers as
+// (A0 + A1) + (A2 + A3); wavefront w then writes dL/dphi of the row's
+// vector chunks w, w + 4, ... and adds their dL/dbeta terms into the
+// workgroup's partial row (each element always by the same lane).  The forward adds the wavefronts' log-sums in wavefront order.
+// Dynamic LDS: f1, q (the source row is staged through it, read
+// coalesced) and, for the adjoint, pi_s and one more combine buffer, each
+// NCH*64*V Floats, and 2 C.
+template <typename F, int NCH, bool BWD>
 __global__ __launch_bounds__(kLlRowWaves * 64) void k_loglik_rows(LogLikArgs a) {
   constexpr int V = VT<F>::V;
   constexpr int W = kLlRowWaves;
-  constexpr int KP = NCH * 64 * V;
-  typedef typename VT<F>::type T;
-  __shared__ __align__(16) F s_pi[KP];
-  __shared__ __align__(16) F s_f1[KP];
+  constexpr int E = NCH * V;  // elements per lane
+  constexpr int KP = E * 64;
+  // two rows in flight up to 8 KB rows; wider rows leave the registers to
+  // q and A (with q read from LDS for each pair instead, the rows in flight
+  // per SIMD do not grow: two rows but one wavefront)
+  constexpr bool DB = NCH <= 8;
+  static_assert(W == 4, "the adjoint's combine tree");
+  extern __shared__ __align__(16) unsigned char s_dynb[];
+  F* const s_f1 = reinterpret_cast<F*>(s_dynb);  // f1_k = beta_k - eps, 0 past K
+  F* const s_q = s_f1 + KP;                       // q_k; adjoint: then a combine buffer
+  F* const s_pk = s_q + KP;                       // adjoint: pi_sk
+  F* const s_b1 = s_pk + KP;                      // adjoint: combine buffer
+  F* const s_c = s_b1 + KP;                       // adjoint: [2]
   __shared__ double s_red[W];
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -2412,194 +2456,200 @@ __global__ __launch_bounds__(kLlRowWaves * 64) void k_loglik_rows(LogLikArgs a) 
     if (k < K) bk = a.theta[2 * k + 1] / (a.theta[2 * k] + a.theta[2 * k + 1]);
     s_f1[k] = k < K ? (F)bk - eps : (F)0;
   }
+  // the adjoint's partial dL/dbeta row: vector chunk c of every source is
+  // added by wavefront c mod 4, each element by the same lane
+  double* const dbr = BWD ? a.dbeta_partials + (int64_t)blockIdx.x * K : nullptr;
+  if constexpr (BWD) {
+    for (int c = w; c < NCH; c += W)
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const int k = (c * 64 + lane) * V + v;
+        if (k < K) dbr[k] = 0.0;
+      }
+  }
+  const double go = BWD ? a.gout[0] : 0.0;
+  // each lane's byte offsets in a neighbour row (a vector past the row
+  // re-reads vector 0; q is 0 there)
+  uint32_t voff[NCH];
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    const int k0 = (c * 64 + lane) * V;
+    voff[c] = (uint32_t)((k0 < K ? k0 : 0) * (int)sizeof(F));
+  }
+  const uint32_t row_bytes = (uint32_t)(a.ld * (int64_t)sizeof(F));
   for (int64_t s = blockIdx.x; s < a.M; s += gridDim.x) {
     const int32_t node = a.nodes[s];
     const F* phis = (const F*)a.phi + s * a.phi_ld;
+    // the source row, read coalesced: sum(phi_s) (per thread, per
+    // wavefront, then in wavefront order), pi_s = phi_s / sum(phi_s), q and
+    // S = sum(pi_s) the same way
+    constexpr int KT = (KP + W * 64 - 1) / (W * 64);  // elements per thread
+    F ph[KT];
     double part = 0.0;
-    for (int k = tid; k < KP; k += W * 64) {
-      const F v = k < K ? phis[k] : (F)0;
-      s_pi[k] = v;
-      part += (double)v;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      const int k = tid + i * W * 64;
+      ph[i] = k < K ? phis[k < K ? k : 0] : (F)0;
+      part += (double)ph[i];
     }
     part = group_sum<64>(part);
+    __syncthreads();  // the previous source's reads of s_red, s_q and s_c are done
     if (lane == 0) s_red[w] = part;
     __syncthreads();
     double sp = 0.0;
-    for (int q = 0; q < W; ++q) sp += s_red[q];  // sum(phi_s), fixed order
-    for (int k = tid; k < KP; k += W * 64) s_pi[k] = (F)((double)s_pi[k] / sp);
+    for (int q2 = 0; q2 < W; ++q2) sp += s_red[q2];  // sum(phi_s)
+    double spi = 0.0;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      const int k = tid + i * W * 64;
+      if (k < KP) {
+        const F pk = k < K ? (F)((double)ph[i] / sp) : (F)0;
+        spi += (double)pk;
+        s_q[k] = pk * s_f1[k];
+        if constexpr (BWD) s_pk[k] = pk;
+      }
+    }
+    spi = group_sum<64>(spi);
+    __syncthreads();  // every wavefront has read s_red
+    if (lane == 0) s_red[w] = spi;
     __syncthreads();
+    double st = 0.0;
+    for (int q2 = 0; q2 < W; ++q2) st += s_red[q2];
+    const F S = (F)st;
+    const F zs1 = e1 * S, zs0 = e0 * S;
+    F q[E];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) load_vec<F>(s_q + (c * 64 + lane) * V, q + c * V);
     const int64_t p0 = min(max(a.pair_ptr[s], (int64_t)0), a.P);
     const int64_t p1 = min(max(a.pair_ptr[s + 1], p0), a.P);
     double ll = 0.0;
-    for (int64_t p = p0 + w; p < p1; p += W) {
-      const int32_t b = __builtin_amdgcn_readfirstlane(a.pair_nbr[p]);
-      const int y = __builtin_amdgcn_readfirstlane(a.pair_y[p] ? 1 : 0);
-      const bool bad = b < 0 || b >= a.N;
-      if (bad && lane == 0) atomicOr(a.error, ERR_BAD_INPUT);
-      const bool ok = !bad && b != node;  // mcmc_sampler_stochastic.cc:275-277
-      const F* row = pi + (int64_t)(ok ? b : 0) * a.ld;
-      F r[NCH * V];
+    F A[BWD ? E : 1];
 #pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-        const int k0 = (c * 64 + lane) * V;
-        load_vec_nt<F>(row + (k0 < K ? k0 : 0), r + c * V);
+    for (int j = 0; j < (BWD ? E : 1); ++j) A[j] = (F)0;
+    F C = (F)0;
+    // this wavefront's pairs: p0 + w + W j, in chunks of 64
+    for (int64_t cb = p0 + w; cb < p1; cb += 64 * W) {
+      const int n = (int)min((int64_t)64, (p1 - cb + W - 1) / W);
+      int32_t idv = 0, yv = 0;
+      {
+        const int64_t pl = cb + (int64_t)lane * W;
+        const int64_t pc = pl < p1 ? pl : cb;
+        idv = a.pair_nbr[pc];
+        yv = a.pair_y[pc] ? 1 : 0;
       }
-      F z = (F)0;
+      // request the j-th pair's row
+      auto fetch = [&](int j, F* r, int& y, bool& ok) {
+        const int32_t b = __builtin_amdgcn_readlane(idv, j);
+        y = __builtin_amdgcn_readlane(yv, j);
+        const bool bad = b < 0 || b >= a.N;
+        if (bad && lane == 0) atomicOr(a.error, ERR_BAD_INPUT);
+        ok = !bad && b != node;  // mcmc_sampler_stochastic.cc:275-277
+        const __amdgpu_buffer_rsrc_t rr = row_rsrc(pi + (int64_t)(ok ? b : 0) * a.ld, row_bytes);
 #pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-        const int k0 = (c * 64 + lane) * V;
-        const T pv = *reinterpret_cast<const T*>(s_pi + k0);
-        const T fv = *reinterpret_cast<const T*>(s_f1 + k0);
-        const F* pp = reinterpret_cast<const F*>(&pv);
-        const F* ff = reinterpret_cast<const F*>(&fv);
+        for (int c = 0; c < NCH; ++c) load_vec_buf<F, true>(rr, voff[c], r + c * V);
+      };
+      // reduce a requested row (q is 0 past K, so no mask is needed)
+      auto reduce = [&](const F* r, int y, bool ok) {
+        F P = (F)0;
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-          const F x = y ? fma(r[c * V + v], ff[v], e1) : fma(-r[c * V + v], ff[v], e0);
-          z = fma(pp[v], x, z);  // pi is 0 past K
+        for (int j = 0; j < E; ++j) P = fma(q[j], r[j], P);
+        P = group_sum<64>(P);
+        const F z = y ? P + zs1 : zs0 - P;
+        if (!ok) return;  // uniform
+        if constexpr (!BWD) {
+          ll += log((double)z);
+        } else {
+          const F rz = recip(z);
+          const F sw = y ? rz : -rz;
+#pragma unroll
+          for (int j = 0; j < E; ++j) A[j] = fma(r[j], sw, A[j]);
+          C += fma(y ? e1 : e0, rz, (F)-1);
+        }
+      };
+      if constexpr (DB) {
+        F ra[E], rb[E];
+        int ya = 0, yb = 0;
+        bool oka = false, okb = false;
+        fetch(0, ra, ya, oka);
+        for (int j = 0; j < n; j += 2) {
+          if (j + 1 < n) fetch(j + 1, rb, yb, okb);
+          reduce(ra, ya, oka);
+          if (j + 1 >= n) break;
+          if (j + 2 < n) fetch(j + 2, ra, ya, oka);
+          reduce(rb, yb, okb);
+        }
+      } else {
+        for (int j = 0; j < n; ++j) {
+          F r[E];
+          int y;
+          bool ok;
+          fetch(j, r, y, ok);
+          reduce(r, y, ok);
         }
       }
-      z = group_sum<64>(z);
-      if (ok) ll += log((double)z);
     }
-    if (lane == 0) s_red[w] = ll;
-    __syncthreads();
-    if (tid == 0) {
-      double t = 0.0;
-      for (int q = 0; q < W; ++q) t += s_red[q];
-      a.per_source[s] = t;
-    }
-    __syncthreads();  // s_pi and s_red are rewritten by the next source
-  }
-}
-
-// Adjoint of k_loglik_rows: each wavefront streams whole neighbour rows,
-// takes Z of a pair from the whole row, and adds x/Z - 1 into its own
-// [K] array in LDS (no cross-wavefront exchange per pair) and the beta
-// terms into its registers.  At the end of a source the four arrays are
-// added in wavefront order into grad_phi; each wavefront writes its beta
-// partial row (partial rows blockIdx*4 + wavefront).  Dynamic LDS: the
-// source row, the link factors and the four arrays, (2 + 4) * NCH*64*V
-// Floats.
-template <typename F, int NCH>
-__global__ __launch_bounds__(kLlRowWaves * 64) void k_loglik_rows_bwd(LogLikArgs a) {
-  constexpr int V = VT<F>::V;
-  constexpr int W = kLlRowWaves;
-  constexpr int KP = NCH * 64 * V;
-  typedef typename VT<F>::type T;
-  extern __shared__ __align__(16) unsigned char s_dynb[];
-  F* s_pi = reinterpret_cast<F*>(s_dynb);
-  F* s_f1 = s_pi + KP;
-  F* s_acc = s_f1 + KP;  // [W][KP]
-  __shared__ double s_red[W];
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int K = a.K;
-  const F* pi = (const F*)a.pi;
-  const F eps = (F)a.epsilon;
-  const F e1 = eps, e0 = (F)1 - eps;
-  const double go = a.gout[0];
-  F* my_acc = s_acc + w * KP;
-  for (int k = tid; k < KP; k += W * 64) {
-    double bk = 0.0;
-    if (k < K) bk = a.theta[2 * k + 1] / (a.theta[2 * k] + a.theta[2 * k + 1]);
-    s_f1[k] = k < K ? (F)bk - eps : (F)0;
-  }
-  double db[NCH * V];
-#pragma unroll
-  for (int j = 0; j < NCH * V; ++j) db[j] = 0.0;
-  for (int64_t s = blockIdx.x; s < a.M; s += gridDim.x) {
-    const int32_t node = a.nodes[s];
-    const F* phis = (const F*)a.phi + s * a.phi_ld;
-    double part = 0.0;
-    for (int k = tid; k < KP; k += W * 64) {
-      const F v = k < K ? phis[k] : (F)0;
-      s_pi[k] = v;
-      part += (double)v;
-    }
-    part = group_sum<64>(part);
-    if (lane == 0) s_red[w] = part;
-    __syncthreads();
-    double sp = 0.0;
-    for (int q = 0; q < W; ++q) sp += s_red[q];  // sum(phi_s), fixed order
-    for (int k = tid; k < KP; k += W * 64) s_pi[k] = (F)((double)s_pi[k] / sp);
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-      T zero;
-      F* zp = reinterpret_cast<F*>(&zero);
-#pragma unroll
-      for (int v = 0; v < V; ++v) zp[v] = (F)0;
-      *reinterpret_cast<T*>(my_acc + (c * 64 + lane) * V) = zero;
-    }
-    __syncthreads();
-    const int64_t p0 = min(max(a.pair_ptr[s], (int64_t)0), a.P);
-    const int64_t p1 = min(max(a.pair_ptr[s + 1], p0), a.P);
-    for (int64_t p = p0 + w; p < p1; p += W) {
-      const int32_t b = __builtin_amdgcn_readfirstlane(a.pair_nbr[p]);
-      const int y = __builtin_amdgcn_readfirstlane(a.pair_y[p] ? 1 : 0);
-      const bool bad = b < 0 || b >= a.N;
-      if (bad && lane == 0) atomicOr(a.error, ERR_BAD_INPUT);
-      const bool ok = !bad && b != node;  // mcmc_sampler_stochastic.cc:275-277
-      const F* row = pi + (int64_t)(ok ? b : 0) * a.ld;
-      F r[NCH * V];
-#pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-        const int k0 = (c * 64 + lane) * V;
-        load_vec_nt<F>(row + (k0 < K ? k0 : 0), r + c * V);
+    if constexpr (!BWD) {
+      if (lane == 0) s_c[0] = (F)0;  // (unused by the forward)
+      __syncthreads();  // every wavefront has read s_red
+      if (lane == 0) s_red[w] = ll;
+      __syncthreads();
+      if (tid == 0) {
+        double t = 0.0;
+        for (int q2 = 0; q2 < W; ++q2) t += s_red[q2];
+        a.per_source[s] = t;
       }
-      F z = (F)0;
+    } else {
+      // (A0 + A1) + (A2 + A3) in wavefront 0; s_q serves as a buffer
+      F* const s_b0 = s_q;
+      auto put = [&](F* buf, int slot) {
 #pragma unroll
-      for (int c = 0; c < NCH; ++c) {
+        for (int c = 0; c < NCH; ++c) store_vec<F>(buf + (c * 64 + lane) * V, A + c * V);
+        if (lane == 0) s_c[slot] = C;
+      };
+      auto add = [&](const F* buf, int slot) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+          F t[V];
+          load_vec<F>(buf + (c * 64 + lane) * V, t);
+#pragma unroll
+          for (int v = 0; v < V; ++v) A[c * V + v] += t[v];
+        }
+        C += s_c[slot];
+      };
+      __syncthreads();  // every wavefront has read s_q
+      if (w == 1) put(s_b0, 0);
+      if (w == 3) put(s_b1, 1);
+      __syncthreads();
+      if (w == 0) add(s_b0, 0);
+      if (w == 2) add(s_b1, 1);
+      __syncthreads();
+      if (w == 2) put(s_b0, 0);
+      __syncthreads();
+      if (w == 0) {
+        add(s_b0, 0);
+        put(s_b1, 1);
+      }
+      __syncthreads();
+      // the sums in s_b1 and s_c[1]; wavefront w finishes chunks w, w+4, ...
+      const F Ct = s_c[1];
+      F* gp = (F*)a.grad_phi + s * a.phi_ld;
+      for (int c = w; c < NCH; c += W) {
         const int k0 = (c * 64 + lane) * V;
-        const T pv = *reinterpret_cast<const T*>(s_pi + k0);
-        const T fv = *reinterpret_cast<const T*>(s_f1 + k0);
-        const F* pp = reinterpret_cast<const F*>(&pv);
-        const F* ff = reinterpret_cast<const F*>(&fv);
+        F f[V], pk[V], At[V];
+        load_vec<F>(s_f1 + k0, f);
+        load_vec<F>(s_pk + k0, pk);
+        load_vec<F>(s_b1 + k0, At);
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-          const F x = y ? fma(r[c * V + v], ff[v], e1) : fma(-r[c * V + v], ff[v], e0);
-          z = fma(pp[v], x, z);  // pi is 0 past K
+          const int k = k0 + v;
+          if (k < K) {
+            gp[k] = (F)(go * (double)fma(f[v], At[v], Ct) / sp);
+            dbr[k] += (double)(pk[v] * At[v]);
+          }
         }
       }
-      z = group_sum<64>(z);
-      if (!ok) continue;  // uniform
-      const F rz = recip(z);
-      const double sg = y ? 1.0 : -1.0;
-#pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-        const int k0 = (c * 64 + lane) * V;
-        const T pv = *reinterpret_cast<const T*>(s_pi + k0);
-        const T fv = *reinterpret_cast<const T*>(s_f1 + k0);
-        T av = *reinterpret_cast<const T*>(my_acc + k0);
-        const F* pp = reinterpret_cast<const F*>(&pv);
-        const F* ff = reinterpret_cast<const F*>(&fv);
-        F* aa = reinterpret_cast<F*>(&av);
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-          const F rv = r[c * V + v];
-          const F x = y ? fma(rv, ff[v], e1) : fma(-rv, ff[v], e0);
-          aa[v] += fma(x, rz, (F)-1);
-          db[c * V + v] += sg * (double)(pp[v] * rv * rz);
-        }
-        *reinterpret_cast<T*>(my_acc + k0) = av;
-      }
     }
-    __syncthreads();
-    F* gp = (F*)a.grad_phi + s * a.phi_ld;
-    for (int k = tid; k < K; k += W * 64) {
-      F t = s_acc[k];
-      for (int q = 1; q < W; ++q) t += s_acc[q * KP + k];
-      gp[k] = (F)(go * (double)t / sp);
-    }
-    __syncthreads();  // s_pi, s_acc and s_red are rewritten by the next source
   }
-  double* out = a.dbeta_partials + ((int64_t)blockIdx.x * W + w) * K;
-#pragma unroll
-  for (int c = 0; c < NCH; ++c)
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const int k = (c * 64 + lane) * V + v;
-      if (k < K) out[k] = db[c * V + v];
-    }
 }
 
 __global__ __launch_bounds__(256) void k_sum(const double* x, int64_t n, double* out) {
@@ -2648,10 +2698,32 @@ int loglik_kc_waves(int32_t K, int precision) {
   return w <= kLlMaxWaves ? w : 0;
 }
 
-size_t loglik_rows_bwd_lds(int32_t K, int precision) {
-  // the source row, the link factors and one adjoint array per wavefront:
-  // (2 + kLlRowWaves) arrays of NCH 16-byte vectors per lane
-  return (size_t)(2 + kLlRowWaves) * loglik_rows_chunks(K, precision) * 64 * 16;
+size_t loglik_rows_lds(int32_t K, int precision, bool backward) {
+  // f1 and q, and for the adjoint pi_s and a combine buffer: NCH 16-byte
+  // vectors per lane each, and 2 C
+  const size_t row = (size_t)loglik_rows_chunks(K, precision) * 64 * 16;
+  return (backward ? 4 : 2) * row + 2 * (precision == 0 ? 8 : 4);
+}
+
+int loglik_rows_blocks_per_cu(int32_t K, int precision) {
+  const int nch = loglik_rows_chunks(K, precision);
+  const size_t lds = loglik_rows_lds(K, precision, true);
+  const void* fn = nullptr;
+#define RB(NC)                                                                       \
+  if (nch == NC)                                                                     \
+    fn = precision == 0 ? (const void*)k_loglik_rows<double, NC, true>               \
+                        : (const void*)k_loglik_rows<float, NC, true>;
+  RB(4)
+  RB(8)
+  RB(16)
+#undef RB
+  if (!fn) return 1;
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+    return 1;
+  int nb = 0;
+  const hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kLlRowWaves * 64, lds);
+  return e == hipSuccess && nb > 0 ? nb : 1;
 }
 
 int loglik_rows_chunks(int32_t K, int precision) {
@@ -2673,47 +2745,32 @@ hipError_t launch_loglik(const LogLikArgs& a, int precision, bool backward, hipS
   if (!row_config(a.K, precision, &lpr, &ch)) return hipErrorInvalidValue;
   if (a.M <= 0) return hipSuccess;
   const int nch = loglik_rows_chunks(a.K, precision);
-  if (rows && backward && nch > 0) {
-    // a.blocks workgroups: a.blocks * kLlRowWaves beta partial rows
-    const size_t lds = loglik_rows_bwd_lds(a.K, precision);
-    const dim3 grid(a.blocks), block(kLlRowWaves * 64);
-#define RB(NC)                                                                          \
-  if (nch == NC) {                                                                      \
-    if (precision == 0) {                                                               \
-      hipError_t e = hipFuncSetAttribute((const void*)k_loglik_rows_bwd<double, NC>,   \
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,    \
-                                         (int)lds);                                     \
-      if (e != hipSuccess) return e;                                                    \
-      hipLaunchKernelGGL((k_loglik_rows_bwd<double, NC>), grid, block, lds, s, a);      \
-    } else {                                                                            \
-      hipError_t e = hipFuncSetAttribute((const void*)k_loglik_rows_bwd<float, NC>,    \
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,    \
-                                         (int)lds);                                     \
-      if (e != hipSuccess) return e;                                                    \
-      hipLaunchKernelGGL((k_loglik_rows_bwd<float, NC>), grid, block, lds, s, a);       \
-    }                                                                                   \
-    return hipGetLastError();                                                           \
+  if (rows && nch > 0) {
+    // backward: a.blocks workgroups (a.blocks beta partial rows); forward:
+    // up to 2048, over the sources
+    const size_t lds = loglik_rows_lds(a.K, precision, backward);
+    const dim3 grid(backward ? (unsigned)a.blocks : (unsigned)std::min<int64_t>(a.M, 2048)),
+        block(kLlRowWaves * 64);
+#define RB(NC, B)                                                                             \
+  if (nch == NC && backward == B) {                                                          \
+    const void* fn = precision == 0 ? (const void*)k_loglik_rows<double, NC, B>              \
+                                    : (const void*)k_loglik_rows<float, NC, B>;              \
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,       \
+                                       (int)lds);                                            \
+    if (e != hipSuccess) return e;                                                           \
+    if (precision == 0)                                                                      \
+      hipLaunchKernelGGL((k_loglik_rows<double, NC, B>), grid, block, lds, s, a);            \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_loglik_rows<float, NC, B>), grid, block, lds, s, a);             \
+    return hipGetLastError();                                                                \
   }
-    RB(4)
-    RB(8)
-    RB(16)
+    RB(4, false)
+    RB(8, false)
+    RB(16, false)
+    RB(4, true)
+    RB(8, true)
+    RB(16, true)
 #undef RB
-    return hipErrorInvalidValue;
-  }
-  if (rows && !backward && nch > 0) {
-    const dim3 grid((unsigned)std::min<int64_t>(a.M, 2048)), block(kLlRowWaves * 64);
-#define RW(NC)                                                                   \
-  if (nch == NC) {                                                               \
-    if (precision == 0)                                                          \
-      hipLaunchKernelGGL((k_loglik_rows<double, NC>), grid, block, 0, s, a);     \
-    else                                                                         \
-      hipLaunchKernelGGL((k_loglik_rows<float, NC>), grid, block, 0, s, a);      \
-    return hipGetLastError();                                                    \
-  }
-    RW(4)
-    RW(8)
-    RW(16)
-#undef RW
     return hipErrorInvalidValue;
   }
   const int waves = loglik_kc_waves(a.K, precision);

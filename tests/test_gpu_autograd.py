@@ -10,8 +10,9 @@ Pins:
     central finite differences on every coordinate
   * FP32 precision mode against FP64 to 1e-4
 K = 48 runs the one-wavefront-per-row kernel; K = 600 (ragged last chunk)
-and K = 2048 run the K-chunked kernel (several wavefronts per row, Z
-exchanged through LDS).
+and K = 2048 run the whole-row kernels (one neighbour row per wavefront),
+and test_kchunk_launch_shapes_agree runs the K-chunked ones (several
+wavefronts per row, Z exchanged through LDS) against them.
 """
 import numpy as np
 import pytest
@@ -200,9 +201,11 @@ def test_argument_checks_before_native_calls(setup):
 
 def test_kchunk_launch_shapes_agree(setup):
     """Every (ll_pairs, ll_prefetch) shape of the K-chunked kernels sums in
-    the same order, so values and gradients are bit-identical; the forward
-    with whole rows per wavefront (the default for wide rows) agrees with the
-    K-chunked forward to 1e-12."""
+    the same order, so values and gradients are bit-identical; the whole-row
+    kernels (the default for wide rows) agree with the K-chunked ones to
+    1e-12 (FP64) or 1e-5 (FP32) of the largest value: they form Z in another
+    order (the regrouped sums), so a source whose log-likelihood is near 0
+    differs by the rounding of the larger terms."""
     g, s, pi, theta, nodes, ptr, nbr, y = setup
     if theta.shape[0] < 600:
         pytest.skip("one wavefront per row at this K")
@@ -226,7 +229,7 @@ def test_kchunk_launch_shapes_agree(setup):
             if ref is None:
                 ref = got
                 tol = 1e-12 if prec == "fp64" else 1e-5
-                np.testing.assert_allclose(rows, got[0], rtol=tol, atol=0)
+                np.testing.assert_allclose(rows, got[0], rtol=tol, atol=tol * np.abs(got[0]).max())
                 for r, o in zip(rows_g, got[1:]):
                     np.testing.assert_allclose(r, o, rtol=tol * 10, atol=tol * np.abs(o).max())
             else:

@@ -6,7 +6,8 @@ Both entry points run the same kernels with the same launch shapes, so
 dL/dphi and dL/dtheta must be bit-identical: on two calls in a row, the
 second with fewer sources and pairs (the array keeps the first call's
 contents), for the one-wavefront-per-row kernel (K 48), the K-chunked kernel
-(FP32 K 1024) and the whole-row kernel (FP64 K 2048).  A call with a
+(FP32 K 1024, selected with ll_bwd_rows = 0) and the whole-row kernel (FP64
+K 2048, the default).  A call with a
 neighbour id outside [0, N) is an error and the next call succeeds (the
 error word in the array is cleared by every call).
 """
@@ -43,8 +44,9 @@ def test_noalloc_adjoint_equals_allocating_adjoint(prec, N, K):
     s = M.make_split(g, 0.1, 42)
     smp = M.MCMCSamplerStochastic(M.Options(K=K, num_node_sample=10, precision=prec), g, s)
     smp.init(5)
-    if K == 2048:
-        assert smp.get_tuning()["ll_bwd_rows"] == 1  # whole-row adjoint (4 partials per block)
+    assert smp.get_tuning()["ll_bwd_rows"] == 1  # whole-row adjoint (one partial row per block)
+    if K == 1024:
+        smp.set_tuning(ll_bwd_rows=0)  # the K-chunked adjoint
     rng = np.random.default_rng(4)
     theta = torch.from_numpy(rng.gamma(2.0, 0.5, size=(K, 2))).cuda()
     for M_src, nnl in ((48, 40), (9, 7)):  # the second call is smaller
