@@ -304,8 +304,8 @@ __device__ __forceinline__ int hash_find(const DevGraph& g, uint32_t lo, uint32_
   const uint64_t key = edge_key(lo, hi);
   uint64_t s = mix64(key) & g.hmask;
   while (true) {
-    const uint64_t k = g.hkeys[s];
-    if (k == key) return g.hvals[s];
+    const uint64_t k = ld_g(g.hkeys + s);
+    if (k == key) return ld_g(g.hvals + s);
     if (k == kHashEmpty) return -1;
     s = (s + 1) & g.hmask;
   }
@@ -316,20 +316,23 @@ __device__ __forceinline__ int hash_find(const DevGraph& g, uint32_t lo, uint32_
 __device__ __forceinline__ bool pair_held_out(const DevGraph& g, uint32_t lo, uint32_t hi) {
   const uint64_t h = bloom_hash(edge_key(lo, hi));
   const uint64_t bits = bloom_bits(h);
-  if ((g.bloom[h & g.bloom_mask] & bits) != bits) return false;
+  if ((ld_g(g.bloom + (h & g.bloom_mask)) & bits) != bits) return false;
   return hash_find(g, lo, hi) >= 0;
 }
 
+// v in the sorted row (binary search); G: the row is in device memory
+template <bool G = false>
 __device__ __forceinline__ bool row_contains(const int32_t* row, int64_t n, int32_t v) {
+  auto at = [&](int64_t i) { return G ? ld_g(row + i) : row[i]; };
   int64_t lo = 0, hi = n;
   while (lo < hi) {
     const int64_t mid = (lo + hi) >> 1;
-    if (row[mid] < v)
+    if (at(mid) < v)
       lo = mid + 1;
     else
       hi = mid;
   }
-  return lo < n && row[lo] == v;
+  return lo < n && at(lo) == v;
 }
 
 // De-dup table: entry = (value, smallest attempt index that drew it).
@@ -498,7 +501,7 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
           (int32_t)randint_fast(draw_u64(a.key, ai, 0, t32, TAG_NONLINK), N, a.g.n_recip);
       bool valid = v != node;
       const uint32_t lo = (uint32_t)min(node, v), hi = (uint32_t)max(node, v);
-      if (valid) valid = !(in_lds ? row_contains(s_row, deg, v) : row_contains(a.g.cols + rs, deg, v));
+      if (valid) valid = !(in_lds ? row_contains(s_row, deg, v) : row_contains<true>(a.g.cols + rs, deg, v));
       if (valid) valid = !pair_held_out(a.g, lo, hi);
       int slot = -1;
       if (valid) {
@@ -769,42 +772,53 @@ __global__ __launch_bounds__(kMbThreads) void k_minibatch(SamplerArgs a) {
 // n+1 accepted (mcmc_sampler_stochastic.h:93-107: n+1 draws, links allowed).
 // Output order = acceptance order; bit 31 flags a link (the y_ab of
 // update_phi, mcmc_sampler_stochastic.cc:279-283).
-template <int WAVES>
+// LDS: the de-dup table (and the node's row) in LDS, else the table in
+// device memory; either way private to the wavefront and ordered by a
+// workgroup-scope fence.  The accesses name their memory (LDS or global)
+// so that none is a flat access (a pointer read from a batch's argument
+// array is generic to the compiler).
+template <int WAVES, bool LDS>
 __device__ __forceinline__ void neighbors_body(const NeighborArgs& a, int32_t slot_offset,
                                                int32_t slot_end, int32_t rslot0, int bx, int by) {
   extern __shared__ int2 s_tab[];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int rslot = rslot0 + by;  // ring slot (iteration)
   const int slot = slot_offset + bx * WAVES + w;  // node slot
-  const IterMeta& mt = a.ring.meta[rslot];
+  const IterMeta* mt = a.ring.meta + rslot;
   // the node id is read together with the slot's count (its index clamped
   // into the slot), not after the count test: one dependent load less
   const int32_t node_any =
-      a.ring.nodes[(int64_t)rslot * a.ring.node_cap + min(slot, (int32_t)a.ring.node_cap - 1)];
-  const int count = mt.node_count;
+      ld_g(a.ring.nodes + (int64_t)rslot * a.ring.node_cap + min(slot, (int32_t)a.ring.node_cap - 1));
+  const int count = ld_g(&mt->node_count);
   if (slot >= count || slot >= slot_end) return;
-  const uint32_t t32 = (uint32_t)mt.step;
+  const uint32_t t32 = (uint32_t)ld_g(&mt->step);
   const uint32_t N = (uint32_t)a.g.N;
   const int32_t node = node_any;
   // LDS per wavefront: the de-dup table, then room for the node's row
-  int2* tab = a.use_lds ? s_tab + (size_t)w * (a.tcap + kNbrRowLds / 2)
-                        : a.gtable + (size_t)(slot - slot_offset) * a.tcap;
-  // The table is private to this wavefront, in LDS or in HBM: ordered by
-  // a workgroup-scope fence (wg_fence) either way.
-  for (int i = lane; i < a.tcap; i += 64) tab[i] = make_int2(-1, INT_MAX);
+  int2* const ltab = s_tab + (size_t)w * (a.tcap + kNbrRowLds / 2);
+  // in device memory: entry i at ints 2i (value) and 2i + 1 (attempt)
+  typedef int gi2 __attribute__((ext_vector_type(2)));
+  const gptr<int> gtab = (gptr<int>)(a.gtable + (size_t)(slot - slot_offset) * a.tcap);
+  for (int i = lane; i < a.tcap; i += 64) {
+    if constexpr (LDS)
+      ltab[i] = make_int2(-1, INT_MAX);
+    else
+      ((gptr<gi2>)gtab)[i] = gi2{-1, INT_MAX};
+  }
   wg_fence();
   __builtin_amdgcn_wave_barrier();
   const uint32_t cmask = (uint32_t)(a.tcap - 1);
-  const int64_t rs = a.g.row_ptr[node];
-  const int64_t deg = a.g.row_ptr[node + 1] - rs;
+  const int64_t rs = ld_g(a.g.row_ptr + node);
+  const int64_t deg = ld_g(a.g.row_ptr + node + 1) - rs;
   const int nnb = a.ring.nnb;
-  int32_t* out = a.ring.nbrs + ((int64_t)rslot * a.ring.node_cap + slot) * nnb;
+  const gptr<int32_t> out =
+      (gptr<int32_t>)(a.ring.nbrs + ((int64_t)rslot * a.ring.node_cap + slot) * nnb);
   // The node's adjacency row in LDS (up to kNbrRowLds entries) for the
   // link test of the accepted draws; longer rows are searched in HBM.
-  const bool row_in_lds = a.use_lds && deg <= kNbrRowLds;
-  int32_t* srow = reinterpret_cast<int32_t*>(tab + a.tcap);
+  const bool row_in_lds = LDS && deg <= kNbrRowLds;
+  int32_t* const srow = reinterpret_cast<int32_t*>(ltab + a.tcap);
   if (row_in_lds) {
-    for (int64_t q = lane; q < deg; q += 64) srow[q] = a.g.cols[rs + q];
+    for (int64_t q = lane; q < deg; q += 64) srow[q] = ld_g(a.g.cols + rs + q);
     __threadfence_block();
     __builtin_amdgcn_wave_barrier();
   }
@@ -822,16 +836,37 @@ __device__ __forceinline__ void neighbors_body(const NeighborArgs& a, int32_t sl
     if (valid) valid = !pair_held_out(a.g, (uint32_t)min(node, v), (uint32_t)max(node, v));
     int s = -1;
     if (valid) {
-      s = table_insert(tab, cmask, v);
-      atomicMin(&tab[s].y, (int)ai);
+      if constexpr (LDS) {
+        s = table_insert(ltab, cmask, v);
+        atomicMin(&ltab[s].y, (int)ai);
+      } else {
+        uint32_t h = (uint32_t)mix64((uint64_t)(uint32_t)v) & cmask;
+        while (true) {
+          int prev = -1;
+          __hip_atomic_compare_exchange_strong(gtab + 2 * h, &prev, v, __ATOMIC_RELAXED,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (prev == -1 || prev == v) break;
+          h = (h + 1) & cmask;
+        }
+        s = (int)h;
+        __hip_atomic_fetch_min(gtab + 2 * s + 1, (int)ai, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
     wg_fence();
     __builtin_amdgcn_wave_barrier();
-    const bool win = valid && (a.use_lds ? tab[s].y : load_agent(&tab[s].y)) == (int)ai;
+    bool win = false;
+    if (valid) {
+      if constexpr (LDS)
+        win = ltab[s].y == (int)ai;
+      else
+        win = __hip_atomic_load(gtab + 2 * s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+              (int)ai;
+    }
     const uint64_t mk = __ballot(win);
     const int pos = c + __popcll(mk & ((1ull << lane) - 1ull));
     if (win && pos < nnb) {
-      const bool lk = row_in_lds ? row_contains(srow, deg, v) : row_contains(a.g.cols + rs, deg, v);
+      const bool lk = row_in_lds ? row_contains(srow, deg, v) : row_contains<true>(a.g.cols + rs, deg, v);
       out[pos] = v | (lk ? (int32_t)0x80000000 : 0);
     }
     c += __popcll(mk);
@@ -841,7 +876,10 @@ __device__ __forceinline__ void neighbors_body(const NeighborArgs& a, int32_t sl
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void k_neighbors(NeighborArgs a, int32_t slot_offset,
                                                           int32_t slot_end, int32_t rslot0) {
-  neighbors_body<WAVES>(a, slot_offset, slot_end, rslot0, blockIdx.x, blockIdx.y);
+  if (a.use_lds)
+    neighbors_body<WAVES, true>(a, slot_offset, slot_end, rslot0, blockIdx.x, blockIdx.y);
+  else
+    neighbors_body<WAVES, false>(a, slot_offset, slot_end, rslot0, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -3152,7 +3190,10 @@ __global__ __launch_bounds__(kMbThreads) void k_minibatch_batch(const SamplerArg
 __global__ __launch_bounds__(kNbrWaves * 64) void k_neighbors_batch(
     const NeighborArgs* __restrict__ args, int slot0, int fills, int slots) {
   const int c = blockIdx.y / fills, j = blockIdx.y % fills;
-  neighbors_body<kNbrWaves>(args[c], 0, INT_MAX, 0, blockIdx.x, (slot0 + j) % slots);
+  if (args[c].use_lds)
+    neighbors_body<kNbrWaves, true>(args[c], 0, INT_MAX, 0, blockIdx.x, (slot0 + j) % slots);
+  else
+    neighbors_body<kNbrWaves, false>(args[c], 0, INT_MAX, 0, blockIdx.x, (slot0 + j) % slots);
 }
 
 template <typename F>
