@@ -1552,70 +1552,111 @@ __global__ __launch_bounds__(WAVES * 64) void k_ppx(PpxArgs a) {
   const double cnt = a.avg_count;
   // Each workgroup takes a contiguous range of the held-out list (sorted by
   // the first endpoint), so the pairs that share row a run on one CU and
-  // find it in that XCD's L2.  Two held-out edges per row group in flight:
-  // ids, labels and running averages first, then all four rows, then the
-  // reductions.
-  constexpr int UE = CH <= 4 ? 2 : 1;  // big rows: one edge at a time (registers)
+  // find it in that XCD's L2.  Row group g of wavefront w takes edges
+  // base + g + i kPer (base = range start + w RPW); the ids, labels and
+  // running averages of its next 64/RPW edges are read at once (lane j:
+  // group j mod RPW, edge j / RPW), so requesting an edge's rows never waits
+  // for a load, and the new running averages are kept in the same layout
+  // and written together after those edges (a store in the loop would be
+  // waited for with the next load).  UE edges per row group in flight, both
+  // rows of each requested before the reductions: 4 for 4 vectors per lane
+  // (C3 FP32: 7.39 -> 7.20 ms per evaluation against 2, two wavefronts per
+  // SIMD instead of three), 2 up to 8 vectors, else 1 (registers).
+  constexpr int UE = CH == 4 ? 4 : (CH <= 8 ? 2 : 1);
   constexpr int64_t kPer = (int64_t)WAVES * RPW;
+  constexpr int IPL = 64 / RPW;  // edges per row group and preload
   const int64_t chunk = ((a.H + a.blocks - 1) / a.blocks + kPer - 1) / kPer * kPer;
   const int64_t e_lo = (int64_t)blockIdx.x * chunk;
   const int64_t e_hi = min(a.H, e_lo + chunk);
-  for (int64_t e0 = e_lo + w * RPW + g; e0 < e_hi; e0 += UE * kPer) {
-    int32_t ea[UE], eb[UE];
-    int yv[UE];
-    bool val[UE];
-    double rold[UE];
-#pragma unroll
-    for (int u = 0; u < UE; ++u) {
-      const int64_t e = e0 + u * kPer;
-      val[u] = e < e_hi;
-      const int64_t ee = val[u] ? e : e0;
-      ea[u] = a.edges[2 * ee];
-      eb[u] = a.edges[2 * ee + 1];
-      yv[u] = a.labels[ee] != 0;
-      rold[u] = lg == 0 ? a.R[ee] : 0.0;
+  const int64_t base = e_lo + w * RPW;
+  auto take_i = [&](int32_t v, int src) -> int32_t {
+    if constexpr (RPW == 1)
+      return __builtin_amdgcn_readlane(v, src);
+    else
+      return __shfl(v, src, 64);
+  };
+  for (int64_t i0 = 0; base + i0 * kPer < e_hi; i0 += IPL) {
+    int32_t pa, pb, py, prl, prh;
+    {
+      const int64_t e = base + lane % RPW + (i0 + lane / RPW) * kPer;
+      const int64_t ee = e < e_hi ? e : e_lo;  // (a valid index; not used past e_hi)
+      pa = a.edges[2 * ee];
+      pb = a.edges[2 * ee + 1];
+      py = a.labels[ee] != 0;
+      const double r = a.R[ee];
+      prl = (int32_t)__double2loint(r);
+      prh = (int32_t)__double2hiint(r);
     }
-    F ra[UE][EPL], rb[UE][EPL];
+    double rnew = 0.0;  // lane j: the new average of its preloaded edge
+    for (int i = 0; i < IPL && base + (i0 + i) * kPer < e_hi; i += UE) {
+      int32_t ea[UE], eb[UE];
+      int yv[UE];
+      bool val[UE];
+      double rold[UE];
+      int64_t ev[UE];
 #pragma unroll
-    for (int u = 0; u < UE; ++u) {
-      if (LPR == 64) {
-        ea[u] = __builtin_amdgcn_readfirstlane(ea[u]);
-        eb[u] = __builtin_amdgcn_readfirstlane(eb[u]);
-        yv[u] = __builtin_amdgcn_readfirstlane(yv[u]);
+      for (int u = 0; u < UE; ++u) {
+        const int src = (i + u) * RPW + g;
+        ev[u] = base + g + (i0 + i + u) * kPer;
+        val[u] = ev[u] < e_hi;
+        ea[u] = take_i(pa, src);
+        eb[u] = take_i(pb, src);
+        yv[u] = take_i(py, src);
+        rold[u] = __hiloint2double(take_i(prh, src), take_i(prl, src));
+        if (!val[u]) ea[u] = eb[u] = 0;  // (row 0: in bounds, unused)
       }
-      issue_row<F, LPR, CH>(pi + (int64_t)ea[u] * a.ld, K, lg, ra[u]);
-      // row a is shared by neighbouring held-out pairs (sorted by a); row b is streamed
-      issue_row<F, LPR, CH, true>(pi + (int64_t)eb[u] * a.ld, K, lg, rb[u]);
-    }
+      F ra[UE][EPL], rb[UE][EPL];
 #pragma unroll
-    for (int u = 0; u < UE; ++u) {
-      if (!val[u]) continue;  // uniform within the row group
-      mask_row<F, LPR, CH>(K, lg, ra[u]);
-      mask_row<F, LPR, CH>(K, lg, rb[u]);
-      const bool y = yv[u] != 0;
-      F s = (F)0, sum = (F)0;
-#pragma unroll
-      for (int j = 0; j < EPL; ++j) {
-        const F f = ra[u][j] * rb[u][j];
-        s += y ? f * bt[j] : f * ((F)1 - bt[j]);
-        sum += f;
+      for (int u = 0; u < UE; ++u) {
+        if (LPR == 64) {
+          ea[u] = __builtin_amdgcn_readfirstlane(ea[u]);
+          eb[u] = __builtin_amdgcn_readfirstlane(eb[u]);
+          yv[u] = __builtin_amdgcn_readfirstlane(yv[u]);
+        }
+        issue_row<F, LPR, CH>(pi + (int64_t)ea[u] * a.ld, K, lg, ra[u]);
+        // row a is shared by neighbouring held-out pairs (sorted by a); row b is streamed
+        issue_row<F, LPR, CH, true>(pi + (int64_t)eb[u] * a.ld, K, lg, rb[u]);
       }
-      s = group_sum<LPR>(s);
-      sum = group_sum<LPR>(sum);
-      if (!y) s += ((F)1 - sum) * ((F)1 - eps);
-      if (s < (F)1e-30) s = (F)1e-30;
-      if (lg == 0) {
+#pragma unroll
+      for (int u = 0; u < UE; ++u) {
+        if (!val[u]) continue;  // uniform within the row group
+        mask_row<F, LPR, CH>(K, lg, ra[u]);
+        mask_row<F, LPR, CH>(K, lg, rb[u]);
+        const bool y = yv[u] != 0;
+        F s = (F)0, sum = (F)0;
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) {
+          const F f = ra[u][j] * rb[u][j];
+          s += y ? f * bt[j] : f * ((F)1 - bt[j]);
+          sum += f;
+        }
+        s = group_sum<LPR>(s);
+        sum = group_sum<LPR>(sum);
+        if (!y) s += ((F)1 - sum) * ((F)1 - eps);
+        if (s < (F)1e-30) s = (F)1e-30;
+        // every lane of the row group holds s and rold, hence r
         const double r = (rold[u] * (cnt - 1.0) + (double)s) / cnt;
-        a.R[e0 + u * kPer] = r;
-        const double lr = log(r);
-        if (y) {
-          ll1 += lr;
-          n1 += 1.0;
+        if constexpr (RPW == 1) {
+          if (lane == i + u) rnew = r;
         } else {
-          ll0 += lr;
-          n0 += 1.0;
+          const double rv = __shfl(r, (lane % RPW) * LPR, 64);
+          if (lane / RPW == i + u) rnew = rv;
+        }
+        if (lg == 0) {
+          const double lr = log(r);
+          if (y) {
+            ll1 += lr;
+            n1 += 1.0;
+          } else {
+            ll0 += lr;
+            n0 += 1.0;
+          }
         }
       }
+    }
+    {
+      const int64_t e = base + lane % RPW + (i0 + lane / RPW) * kPer;
+      if (e < e_hi) a.R[e] = rnew;
     }
   }
   // lanes with lg != 0 hold zeros
