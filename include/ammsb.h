@@ -439,7 +439,7 @@ int ammsb_batch_use_graph(ammsb_batch *b, int32_t enable);
 /* Sampler refills of the batch's ring (no counterpart in the reference):
  * 0 (default) draws the next 16 iterations on a second stream while the
  * current 16 run; 1 draws them on the batch's stream, in order with the
- * iterations (64 chains on one MI355X: 0.513 against 0.538 ms per
+ * iterations (64 chains on one MI355X: 0.492 against 0.518 ms per
  * iteration).  The draws are the same. */
 int ammsb_batch_refill_in_order(ammsb_batch *b, int32_t enable);
 
