@@ -2668,7 +2668,6 @@ __global__ __launch_bounds__(kLlRowWaves * 64) void k_loglik_rows(LogLikArgs a) 
       }
     }
     if constexpr (!BWD) {
-      if (lane == 0) s_c[0] = (F)0;  // (unused by the forward)
       __syncthreads();  // every wavefront has read s_red
       if (lane == 0) s_red[w] = ll;
       __syncthreads();
