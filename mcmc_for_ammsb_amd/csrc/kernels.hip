@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <limits.h>
 #include <math.h>
+#include <type_traits>
 
 #include "host_util.h"
 #include "kernels.h"
@@ -452,10 +453,12 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
   DevState* st = a.st;
   const uint32_t t32 = (uint32_t)t;
   const Ring& R = a.ring;
-  int32_t* const mb_edges = R.edges + (int64_t)slot * R.edge_cap * 2;
-  uint8_t* const mb_y = R.y + (int64_t)slot * R.edge_cap;
-  int32_t* const out_nodes = R.nodes + (int64_t)slot * R.node_cap;
-  uint32_t* const stamp = R.stamp + (int64_t)slot * R.N;
+  // the ring's arrays, the graph's and the HBM tables are device memory:
+  // global accesses (a batch reads these pointers from an argument array)
+  const gptr<int32_t> mb_edges = (gptr<int32_t>)(R.edges + (int64_t)slot * R.edge_cap * 2);
+  const gptr<uint8_t> mb_y = (gptr<uint8_t>)(R.y + (int64_t)slot * R.edge_cap);
+  const gptr<int32_t> out_nodes = (gptr<int32_t>)(R.nodes + (int64_t)slot * R.node_cap);
+  const gptr<uint32_t> stamp = (gptr<uint32_t>)(R.stamp + (int64_t)slot * R.N);
   const int64_t edge_cap = R.edge_cap, node_cap = R.node_cap;
   const uint32_t N = (uint32_t)a.g.N;
   const int64_t m = a.m;
@@ -480,50 +483,86 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
   if (flag == 0) {
     const int32_t node =
         (int32_t)randint_fast(draw_u64(a.key, 0, 0, t32, TAG_SOURCE), N, a.g.n_recip);
-    const int64_t rs = a.g.row_ptr[node];
-    const int64_t deg = a.g.row_ptr[node + 1] - rs;
+    const int64_t rs = ld_g(a.g.row_ptr + node);
+    const int64_t deg = ld_g(a.g.row_ptr + node + 1) - rs;
     const bool in_lds = deg <= kRowLds;
     if (in_lds)
-      for (int64_t i = tid; i < deg; i += kMbThreads) s_row[i] = a.g.cols[rs + i];
-    int2* tab = a.table_in_lds ? s_dyn : a.table + (int64_t)slot * a.table_cap;
-    for (int64_t i = tid; i < a.table_cap; i += kMbThreads) tab[i] = make_int2(-1, INT_MAX);
-    wg_fence();
-    __syncthreads();
+      for (int64_t i = tid; i < deg; i += kMbThreads) s_row[i] = ld_g(a.g.cols + rs + i);
     const uint32_t cmask = (uint32_t)(a.table_cap - 1);
     const int64_t max_batches = 4096 + 64 * ((m + kMbThreads - 1) / kMbThreads);
-    for (int64_t batch = 0;; ++batch) {
-      if (batch >= max_batches) {
-        err |= ERR_NONLINK_ATTEMPTS;
-        break;
-      }
-      const uint32_t ai = (uint32_t)(batch * kMbThreads + tid);
-      const int32_t v =
-          (int32_t)randint_fast(draw_u64(a.key, ai, 0, t32, TAG_NONLINK), N, a.g.n_recip);
-      bool valid = v != node;
-      const uint32_t lo = (uint32_t)min(node, v), hi = (uint32_t)max(node, v);
-      if (valid) valid = !(in_lds ? row_contains(s_row, deg, v) : row_contains<true>(a.g.cols + rs, deg, v));
-      if (valid) valid = !pair_held_out(a.g, lo, hi);
-      int slot = -1;
-      if (valid) {
-        slot = table_insert(tab, cmask, v);
-        atomicMin(&tab[slot].y, (int)ai);
+    // the de-dup table in LDS (LDS atomics) or in HBM (global atomics;
+    // entry i at ints 2i, the value, and 2i + 1, the attempt)
+    auto nonlinks = [&](auto in_lds_table) {
+      constexpr bool L = decltype(in_lds_table)::value;
+      typedef int gi2 __attribute__((ext_vector_type(2)));
+      int2* const ltab = s_dyn;
+      const gptr<int> gtab = (gptr<int>)(a.table + (int64_t)slot * a.table_cap);
+      for (int64_t i = tid; i < a.table_cap; i += kMbThreads) {
+        if constexpr (L)
+          ltab[i] = make_int2(-1, INT_MAX);
+        else
+          ((gptr<gi2>)gtab)[i] = gi2{-1, INT_MAX};
       }
       wg_fence();
       __syncthreads();
-      const int win =
-          valid && (a.table_in_lds ? tab[slot].y : load_agent(&tab[slot].y)) == (int)ai;
-      int total;
-      const int pos = block_scan(win, s_wave, &total);
-      if (win && c + pos < m) {
-        mb_edges[2 * (c + pos)] = (int32_t)lo;
-        mb_edges[2 * (c + pos) + 1] = (int32_t)hi;
-        mb_y[c + pos] = 0;
-        // node set: the source, then the (distinct) partners in order
-        if (1 + c + pos < node_cap) out_nodes[1 + c + pos] = v;
+      for (int64_t batch = 0;; ++batch) {
+        if (batch >= max_batches) {
+          err |= ERR_NONLINK_ATTEMPTS;
+          break;
+        }
+        const uint32_t ai = (uint32_t)(batch * kMbThreads + tid);
+        const int32_t v =
+            (int32_t)randint_fast(draw_u64(a.key, ai, 0, t32, TAG_NONLINK), N, a.g.n_recip);
+        bool valid = v != node;
+        const uint32_t lo = (uint32_t)min(node, v), hi = (uint32_t)max(node, v);
+        if (valid) valid = !(in_lds ? row_contains(s_row, deg, v) : row_contains<true>(a.g.cols + rs, deg, v));
+        if (valid) valid = !pair_held_out(a.g, lo, hi);
+        int ts = -1;
+        if (valid) {
+          if constexpr (L) {
+            ts = table_insert(ltab, cmask, v);
+            atomicMin(&ltab[ts].y, (int)ai);
+          } else {
+            uint32_t h = (uint32_t)mix64((uint64_t)(uint32_t)v) & cmask;
+            while (true) {
+              int prev = -1;
+              __hip_atomic_compare_exchange_strong(gtab + 2 * h, &prev, v, __ATOMIC_RELAXED,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              if (prev == -1 || prev == v) break;
+              h = (h + 1) & cmask;
+            }
+            ts = (int)h;
+            __hip_atomic_fetch_min(gtab + 2 * ts + 1, (int)ai, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+        wg_fence();
+        __syncthreads();
+        int win = 0;
+        if (valid) {
+          if constexpr (L)
+            win = ltab[ts].y == (int)ai;
+          else
+            win = __hip_atomic_load(gtab + 2 * ts + 1, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT) == (int)ai;
+        }
+        int total;
+        const int pos = block_scan(win, s_wave, &total);
+        if (win && c + pos < m) {
+          mb_edges[2 * (c + pos)] = (int32_t)lo;
+          mb_edges[2 * (c + pos) + 1] = (int32_t)hi;
+          mb_y[c + pos] = 0;
+          // node set: the source, then the (distinct) partners in order
+          if (1 + c + pos < node_cap) out_nodes[1 + c + pos] = v;
+        }
+        c = min(m, c + (int64_t)total);
+        if (c >= m) break;
       }
-      c = min(m, c + (int64_t)total);
-      if (c >= m) break;
-    }
+    };
+    if (a.table_in_lds)
+      nonlinks(std::integral_constant<bool, true>());
+    else
+      nonlinks(std::integral_constant<bool, false>());
     if (tid == 0) {
       if (c > 0) out_nodes[0] = node;
       s_nodes = c > 0 ? (int)c + 1 : 0;
@@ -544,8 +583,8 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
         bool keep = false;
         int32_t u = 0, v = 0;
         if (q < E2) {
-          u = a.g.esrc[q];
-          v = a.g.cols[q];
+          u = ld_g(a.g.esrc + q);
+          v = ld_g(a.g.cols + q);
           keep = u < v && !pair_held_out(a.g, (uint32_t)u, (uint32_t)v);
         }
         int total;
@@ -559,8 +598,9 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
         c = min(edge_cap, c + (int64_t)total);
       }
     } else {
-      unsigned long long* keys = a.re_keys + (int64_t)slot * a.re_cap;
-      int32_t* att = a.re_att + (int64_t)slot * a.re_cap;
+      const gptr<unsigned long long> keys =
+          (gptr<unsigned long long>)(a.re_keys + (int64_t)slot * a.re_cap);
+      const gptr<int32_t> att = (gptr<int32_t>)(a.re_att + (int64_t)slot * a.re_cap);
       for (int64_t i = tid; i < a.re_cap; i += kMbThreads) {
         keys[i] = ~0ull;
         att[i] = INT_MAX;
@@ -577,17 +617,27 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
         const uint32_t ai = (uint32_t)(batch * kMbThreads + tid);
         const int64_t q = (int64_t)randint_fast(draw_u64(a.key, ai, 0, t32, TAG_RANDOM_EDGE),
                                                 (uint32_t)E2, a.g.e2_recip);
-        const int32_t u = a.g.esrc[q], v = a.g.cols[q];
+        const int32_t u = ld_g(a.g.esrc + q), v = ld_g(a.g.cols + q);
         const uint32_t lo = (uint32_t)min(u, v), hi = (uint32_t)max(u, v);
         const bool valid = !pair_held_out(a.g, lo, hi);
         int ts = -1;
         if (valid) {
-          ts = table_insert64(keys, cmask, (unsigned long long)edge_key(lo, hi));
-          atomicMin(&att[ts], (int)ai);
+          const unsigned long long key = (unsigned long long)edge_key(lo, hi);
+          uint32_t h = (uint32_t)mix64(key) & cmask;
+          while (true) {
+            unsigned long long prev = ~0ull;
+            __hip_atomic_compare_exchange_strong(keys + h, &prev, key, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (prev == ~0ull || prev == key) break;
+            h = (h + 1) & cmask;
+          }
+          ts = (int)h;
+          __hip_atomic_fetch_min(att + ts, (int)ai, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         wg_fence();
         __syncthreads();
-        const int win = valid && load_agent(&att[ts]) == (int)ai;
+        const int win = valid && __hip_atomic_load(att + ts, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT) == (int)ai;
         int total;
         const int pos = block_scan(win, s_wave, &total);
         if (win && c + pos < m) {
@@ -614,7 +664,7 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
             (int32_t)randint_fast(draw_u64(a.key, (uint32_t)r, (uint32_t)s, t32, TAG_SOURCE), N,
                                   a.g.n_recip);
         s_src[s] = node;
-        s_deg[s] = (int)(a.g.row_ptr[node + 1] - a.g.row_ptr[node]);
+        s_deg[s] = (int)(ld_g(a.g.row_ptr + node + 1) - ld_g(a.g.row_ptr + node));
         const int slot = table_insert(s_stab, kSrcTab - 1, node);
         atomicMin(&s_stab[slot].y, s);
       }
@@ -631,13 +681,13 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
           const int32_t node = s_src[s];
           int kept = 0;
           if (src_first(s_stab, node) == s) {
-            const int64_t rs = a.g.row_ptr[node];
+            const int64_t rs = ld_g(a.g.row_ptr + node);
             const int deg = s_deg[s];
             for (int j0 = 0; j0 < deg; j0 += 64) {
               const int j = j0 + lane;
               bool keep = false;
               if (j < deg) {
-                const int32_t v = a.g.cols[rs + j];
+                const int32_t v = ld_g(a.g.cols + rs + j);
                 keep = src_first(s_stab, v) >= s &&
                        !pair_held_out(a.g, (uint32_t)min(node, v), (uint32_t)max(node, v));
               }
@@ -676,7 +726,7 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
         for (int s = w; s < used; s += kMbWaves) {
           const int32_t node = s_src[s];
           if (src_first(s_stab, node) != s) continue;
-          const int64_t rs = a.g.row_ptr[node];
+          const int64_t rs = ld_g(a.g.row_ptr + node);
           const int deg = s_deg[s];
           int off = s_cb[s];
           for (int j0 = 0; j0 < deg; j0 += 64) {
@@ -684,7 +734,7 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
             bool keep = false;
             int32_t v = 0;
             if (j < deg) {
-              v = a.g.cols[rs + j];
+              v = ld_g(a.g.cols + rs + j);
               keep = src_first(s_stab, v) >= s &&
                      !pair_held_out(a.g, (uint32_t)min(node, v), (uint32_t)max(node, v));
             }
@@ -732,7 +782,8 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
       if (e < c) {
         v = mb_edges[2 * e + q];
         isnew = lds_set ? set_insert(nset, (uint32_t)(set_ints - 1), v)
-                        : atomicExch(&stamp[v], epoch) != epoch;
+                        : __hip_atomic_exchange(stamp + v, epoch, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT) != epoch;
       }
       // one LDS atomic per wavefront, positions by ballot prefix
       const uint64_t mk = __ballot(isnew);
