@@ -3153,8 +3153,9 @@ __host__ __device__ constexpr size_t batch_layout_lds(int n) {
   return (size_t)n * sizeof(int4) + (size_t)(n + 1) * sizeof(int32_t);
 }
 
-// Resident wavefronts per SIMD: five in FP32 (registers bounded to 94; six
-// would spill), three in FP64 (152 registers; five would spill).
+// Resident wavefronts per SIMD: five in FP32 (registers bounded to 96, one
+// register spilled outside the row loop; six would spill in it), three in
+// FP64 (168 registers).
 template <typename F, int U>
 __global__ __launch_bounds__(kBatchWaves * 64)
 __attribute__((amdgpu_waves_per_eu(sizeof(F) == 4 ? 5 : 3))) void k_update_phi_batch(
