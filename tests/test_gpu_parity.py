@@ -167,8 +167,8 @@ def test_update_beta_fp64_injected(strategy, max_source, K):
     assert ok, rel
 
 
-@pytest.mark.parametrize("K,prec", [(32, "fp64"), (256, "fp64"), (300, "fp64"), (2048, "fp64"),
-                                    (64, "fp32"), (4096, "fp32")])
+@pytest.mark.parametrize("K,prec", [(32, "fp64"), (256, "fp64"), (300, "fp64"), (512, "fp64"),
+                                    (2048, "fp64"), (64, "fp32"), (1024, "fp32"), (4096, "fp32")])
 def test_perplexity_running_average(K, prec):
     print(f"perplexity K={K} {prec}", flush=True)
     N = 5000
