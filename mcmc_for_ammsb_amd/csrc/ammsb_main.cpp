@@ -83,6 +83,9 @@ int main(int argc, char** argv) {
   ammsb_ctx* ctx = nullptr;
   int rc = ammsb_create(&opts, &g, nullptr, &ctx);  // null split: drawn as the reference does
   if (rc == AMMSB_OK) rc = ammsb_init_state(ctx, (uint64_t)(uint32_t)opts.random_seed);
+  // every phase timed, as the reference's Timers are (its table is printed
+  // every --mcmc.stats steps and at the end)
+  if (rc == AMMSB_OK) rc = ammsb_enable_timers(ctx, 1);
   if (rc == AMMSB_OK) rc = ammsb_run(ctx, 0, 1, nullptr, nullptr, 0, nullptr);
   if (rc != AMMSB_OK) fprintf(stderr, "error %d: %s\n", rc, ammsb_last_error());
   if (ctx) ammsb_destroy(ctx);

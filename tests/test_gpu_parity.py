@@ -300,6 +300,26 @@ def test_driver_matches_python_run(tmp_path):
     assert [t for t, _ in got] == [t for t, _ in ref] == [1, 21, 41]
     for (_, a), (_, b) in zip(got, ref):
         assert abs(a - b) <= 1e-11, (a, b)
+    # the timer table printed at the end of run() (PrintStats,
+    # mcmc_sampler_stochastic.cc:185-198): timer.cc's tabular layout with
+    # timer.h's widths (name 36, total 12, ticks 8, per tick 14), fixed
+    # 3-decimal numbers, the eight phases in the reference's order
+    lines = r.stdout.splitlines()
+    head = "timer".ljust(37) + "total (s)".rjust(13) + "ticks".rjust(9) + "per tick (us)".rjust(15)
+    assert head in lines
+    i = lines.index(head)
+    names = ["outer", "perplexity", "sample_mini_batch", "nodes_in_mini_batch",
+             "sample_neighbor_nodes", "update_phi", "update_pi", "update_beta"]
+    row = re.compile(r"^(  [a-z_]+) {1,}( {0,12}\d+\.\d{3}) ( {0,8}\d+) ( {0,14}\d+\.\d{3})$")
+    for k, name in enumerate(names):
+        text = lines[i + 1 + k]
+        assert text[:36] == ("  " + name).ljust(36), text
+        if text[36:] == "<unused>":
+            continue
+        m = row.match(text)
+        assert m and len(text) == 36 + 1 + 12 + 1 + 8 + 1 + 14, text
+    ticks = {n: lines[i + 1 + k][36:] for k, n in enumerate(names)}
+    assert ticks["outer"] != "<unused>" and ticks["update_phi"] != "<unused>"
 
 
 def test_trajectory_n5000_within_one_percent():
