@@ -211,6 +211,51 @@ def test_split_semantics():
         M.make_split(M.generate("community", N=100, K=4, mean_degree=2.0, seed=1), 3.0, 1)
 
 
+def test_split_draws_are_uniform():
+    """The split's draws over many seeds: every training edge is equally
+    likely to be a held-out link and a test link (init_held_out_set /
+    init_test_set sample links uniformly, network.cc:503-580), and every
+    node about equally likely as a held-out non-link endpoint (non-links are
+    uniform pairs, :533-557).  Chi-square statistics within five standard
+    deviations of their degrees of freedom; a split that favoured low ids,
+    CSR order or high-degree nodes would land far outside."""
+    g = M.generate("community", N=200, K=4, mean_degree=6.0, seed=5)
+    E = g.num_edges
+    src = np.repeat(np.arange(g.N), np.diff(g.row_ptr))
+    keep = src < g.cols
+    eid = {(int(a), int(b)): i for i, (a, b) in enumerate(zip(src[keep], g.cols[keep]))}
+    assert len(eid) == E
+    ho = np.zeros(E)
+    te = np.zeros(E)
+    ends = np.zeros(g.N)
+    seeds = 400
+    for seed in range(seeds):
+        s = M.make_split(g, 0.1, 1000 + seed)
+        for (a, b), y in zip(s.held_out_edges, s.held_out_labels):
+            if y:
+                ho[eid[(int(a), int(b))]] += 1
+            else:
+                ends[a] += 1
+                ends[b] += 1
+        for (a, b), y in zip(s.test_edges, s.test_labels):
+            if y:
+                te[eid[(int(a), int(b))]] += 1
+
+    def chi2_ok(counts, expected):
+        x = float((((counts - expected) ** 2) / expected).sum())
+        df = len(counts) - 1
+        return abs(x - df) < 5 * np.sqrt(2 * df), x
+
+    for counts in (ho, te):
+        ok, x = chi2_ok(counts, counts.sum() / E)
+        assert ok, x
+    # a non-link endpoint: node v is in N-1-deg(v) non-link pairs
+    deg = np.diff(g.row_ptr).astype(np.float64)
+    w = (g.N - 1 - deg) / (g.N - 1 - deg).sum()
+    ok, x = chi2_ok(ends, ends.sum() * w)
+    assert ok, x
+
+
 def test_graph_from_edges_and_file(tmp_path):
     """relativity.cc:20-193: 4 header lines, dense renumbering in order of
     first appearance, self-links and repeated pairs dropped, gzip input,
