@@ -346,6 +346,27 @@ __device__ __forceinline__ int table_insert(int2* tab, uint32_t cap_mask, int32_
   }
 }
 
+// The same table in device memory, as ints (entry i at 2i, the value, and
+// 2i + 1, the smallest attempt index that drew it; global atomics): insert
+// v, lower its attempt to ai, return the slot.
+__device__ __forceinline__ int table_insert_min_g(gptr<int> tab, uint32_t cap_mask, int32_t v,
+                                                  int ai) {
+  uint32_t s = (uint32_t)mix64((uint64_t)(uint32_t)v) & cap_mask;
+  while (true) {
+    int prev = -1;
+    __hip_atomic_compare_exchange_strong(tab + 2 * s, &prev, v, __ATOMIC_RELAXED,
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == -1 || prev == v) break;
+    s = (s + 1) & cap_mask;
+  }
+  __hip_atomic_fetch_min(tab + 2 * s + 1, ai, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return (int)s;
+}
+// The attempt index recorded in slot s of a device-memory table.
+__device__ __forceinline__ int table_attempt_g(gptr<int> tab, int s) {
+  return __hip_atomic_load(tab + 2 * s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Hash set of node ids (-1 = empty): true when v was not in the set.
 __device__ __forceinline__ bool set_insert(int* set, uint32_t cap_mask, int32_t v) {
   uint32_t s = (uint32_t)mix64((uint64_t)(uint32_t)v) & cap_mask;
@@ -523,17 +544,7 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
             ts = table_insert(ltab, cmask, v);
             atomicMin(&ltab[ts].y, (int)ai);
           } else {
-            uint32_t h = (uint32_t)mix64((uint64_t)(uint32_t)v) & cmask;
-            while (true) {
-              int prev = -1;
-              __hip_atomic_compare_exchange_strong(gtab + 2 * h, &prev, v, __ATOMIC_RELAXED,
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              if (prev == -1 || prev == v) break;
-              h = (h + 1) & cmask;
-            }
-            ts = (int)h;
-            __hip_atomic_fetch_min(gtab + 2 * ts + 1, (int)ai, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+            ts = table_insert_min_g(gtab, cmask, v, (int)ai);
           }
         }
         wg_fence();
@@ -543,8 +554,7 @@ __device__ __forceinline__ void minibatch_body(const SamplerArgs& a, const int s
           if constexpr (L)
             win = ltab[ts].y == (int)ai;
           else
-            win = __hip_atomic_load(gtab + 2 * ts + 1, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT) == (int)ai;
+            win = table_attempt_g(gtab, ts) == (int)ai;
         }
         int total;
         const int pos = block_scan(win, s_wave, &total);
@@ -891,17 +901,7 @@ __device__ __forceinline__ void neighbors_body(const NeighborArgs& a, int32_t sl
         s = table_insert(ltab, cmask, v);
         atomicMin(&ltab[s].y, (int)ai);
       } else {
-        uint32_t h = (uint32_t)mix64((uint64_t)(uint32_t)v) & cmask;
-        while (true) {
-          int prev = -1;
-          __hip_atomic_compare_exchange_strong(gtab + 2 * h, &prev, v, __ATOMIC_RELAXED,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (prev == -1 || prev == v) break;
-          h = (h + 1) & cmask;
-        }
-        s = (int)h;
-        __hip_atomic_fetch_min(gtab + 2 * s + 1, (int)ai, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+        s = table_insert_min_g(gtab, cmask, v, (int)ai);
       }
     }
     wg_fence();
@@ -911,8 +911,7 @@ __device__ __forceinline__ void neighbors_body(const NeighborArgs& a, int32_t sl
       if constexpr (LDS)
         win = ltab[s].y == (int)ai;
       else
-        win = __hip_atomic_load(gtab + 2 * s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-              (int)ai;
+        win = table_attempt_g(gtab, s) == (int)ai;
     }
     const uint64_t mk = __ballot(win);
     const int pos = c + __popcll(mk & ((1ull << lane) - 1ull));
