@@ -388,8 +388,9 @@ int ammsb_loglik_bwd(ammsb_ctx *ctx, int64_t M, const int32_t *nodes, const void
  * error word -- lives in a file-scope device array of 32 MiB + 4 KiB per GPU,
  * held until the process ends; a call that needs more is AMMSB_ERR_ARG
  * (rows = the adjoint's workgroups, one partial row each: up to 1024 for
- * the K-chunked kernels, the resident workgroups for the whole-row one;
- * at K 2048, 8 MiB or less).  Same arguments, results and errors as
+ * the K-chunked kernels, the resident workgroups for the whole-row one:
+ * at K 2048 on an MI355X 8 MiB in FP64, 12 MiB in FP32).  Same arguments,
+ * results and errors as
  * ammsb_loglik_bwd otherwise.  All calls on one GPU share the array: calls
  * that overlap on one GPU race. */
 int ammsb_loglik_bwd_noalloc(ammsb_ctx *ctx, int64_t M, const int32_t *nodes, const void *phi,
