@@ -7,7 +7,7 @@ dL/dphi and dL/dtheta must be bit-identical: on two calls in a row, the
 second with fewer sources and pairs (the array keeps the first call's
 contents), for the one-wavefront-per-row kernel (K 48), the K-chunked kernel
 (FP32 K 1024, selected with ll_bwd_rows = 0) and the whole-row kernel (FP64
-K 2048, the default).  A call with a
+and FP32 K 2048, the default).  A call with a
 neighbour id outside [0, N) is an error and the next call succeeds (the
 error word in the array is cleared by every call).
 """
@@ -36,7 +36,7 @@ def _bwd(fn, smp, phi, theta, nodes, ptr, nbr, y):
 
 
 @pytest.mark.parametrize("prec,N,K", [("fp64", 20000, 48), ("fp32", 8000, 1024),
-                                      ("fp64", 4000, 2048)])
+                                      ("fp64", 4000, 2048), ("fp32", 4000, 2048)])
 def test_noalloc_adjoint_equals_allocating_adjoint(prec, N, K):
     print(f"no-allocation adjoint {prec} K={K}", flush=True)
     dt = torch.float64 if prec == "fp64" else torch.float32
