@@ -200,14 +200,17 @@ struct ThetaArgs {
   int32_t bump_step;       // 1: st->step += 1 at the end (end of an iteration)
 };
 
+// label of a held-out position that joins two trails (no pair)
+constexpr uint8_t kPpxBridge = 2;
+
 struct PpxArgs {
   const void* pi;
   int64_t ld;
   int32_t K;
-  const int32_t* edges;   // [H][2]
-  const uint8_t* labels;  // [H]
-  int64_t H;
-  double* R;              // running average per held-out edge
+  const int32_t* seq;     // [H + 1]: position t is the pair (seq[t], seq[t + 1])
+  const uint8_t* labels;  // [H]: 0 or 1, kPpxBridge for a position that is no pair
+  int64_t H;              // positions
+  double* R;              // running average per position
   double avg_count;       // Learner::average_count
   const double* beta;
   double epsilon;

@@ -1600,113 +1600,166 @@ __global__ __launch_bounds__(WAVES * 64) void k_ppx(PpxArgs a) {
     }
   double ll1 = 0.0, ll0 = 0.0, n1 = 0.0, n0 = 0.0;
   const double cnt = a.avg_count;
-  // Each workgroup takes a contiguous range of the held-out list (sorted by
-  // the first endpoint), so the pairs that share row a run on one CU and
-  // find it in that XCD's L2.  Row group g of wavefront w takes edges
-  // base + g + i kPer (base = range start + w RPW); the ids, labels and
-  // running averages of its next 64/RPW edges are read at once (lane j:
-  // group j mod RPW, edge j / RPW), so requesting an edge's rows never waits
-  // for a load, and the new running averages are kept in the same layout
-  // and written together after those edges (a store in the loop would be
-  // waited for with the next load).  UE edges per row group in flight, both
-  // rows of each requested before the reductions: 4 for 4 vectors per lane
-  // (C3 FP32: 7.39 -> 7.20 ms per evaluation against 2, two wavefronts per
-  // SIMD instead of three), 2 up to 8 vectors, else 1 (registers).
-  constexpr int UE = CH == 4 ? 4 : (CH <= 8 ? 2 : 1);
-  constexpr int64_t kPer = (int64_t)WAVES * RPW;
-  constexpr int IPL = 64 / RPW;  // edges per row group and preload
-  const int64_t chunk = ((a.H + a.blocks - 1) / a.blocks + kPer - 1) / kPer * kPer;
+  // The held-out pairs are one node sequence, position t the pair
+  // (seq[t], seq[t + 1]) (a bridge position between two trails is no
+  // pair).  Each workgroup takes a contiguous range of positions and each
+  // of its wavefronts a contiguous part of that range.  The ids, labels and
+  // running averages of a wavefront's next 64 positions are read at once
+  // (lane j: position w_lo + i0 RPW + j), so requesting a pair's rows never
+  // waits for a load, and the new running averages are kept in the same
+  // layout and written together after those positions (a store in the loop
+  // would be waited for with the next load).
+  //
+  // One row group per wavefront and up to 8 vectors per lane: the
+  // wavefront walks its part of the sequence UE positions at a time,
+  // requesting the rows seq[t + 1 .. t + UE] into slots 1..UE, while slot 0
+  // holds seq[t] (the last row of the previous batch): every row of the
+  // sequence is requested once, and pair u is slots (u, u + 1).  UE = 8 up
+  // to 4 vectors per lane, else 2 (C3 FP64, 8 vectors: 12.3 ms per
+  // evaluation, 12.2 with UE = 4 at one wavefront per SIMD, 13.9 requesting
+  // both rows of each pair).  Otherwise (several row groups per wavefront
+  // for short rows; for longer rows the slots would cost a wavefront per
+  // SIMD) row group g takes positions w_lo + g + i RPW, both rows of each
+  // requested (the row shared with the previous position is found in L2),
+  // UE pairs in flight.
+  constexpr bool kSeq = LPR == 64 && CH <= 8;
+  constexpr int UE = kSeq ? (CH <= 4 ? 8 : 2) : (CH <= 8 ? 2 : 1);
+  constexpr int64_t kAlign = (int64_t)WAVES * RPW;
+  constexpr int IPL = 64 / RPW;  // positions per row group and preload
+  const int64_t chunk = ((a.H + a.blocks - 1) / a.blocks + kAlign - 1) / kAlign * kAlign;
+  const int64_t wchunk = chunk / WAVES;  // a multiple of RPW
   const int64_t e_lo = (int64_t)blockIdx.x * chunk;
   const int64_t e_hi = min(a.H, e_lo + chunk);
-  const int64_t base = e_lo + w * RPW;
+  const int64_t w_lo = e_lo + w * wchunk;
+  const int64_t w_hi = min(e_hi, w_lo + wchunk);
   auto take_i = [&](int32_t v, int src) -> int32_t {
     if constexpr (RPW == 1)
       return __builtin_amdgcn_readlane(v, src);
     else
       return __shfl(v, src, 64);
   };
-  for (int64_t i0 = 0; base + i0 * kPer < e_hi; i0 += IPL) {
+  double rnew = 0.0;  // lane j: the new average of its preloaded position
+  // the likelihood of one pair from its two (masked) rows, added to the
+  // per-label sums; returns the new running average (held by every lane of
+  // the row group)
+  auto pair_ll = [&](const F (&ra)[EPL], const F (&rb)[EPL], const bool y,
+                     const double rold) -> double {
+    F s = (F)0, sum = (F)0;
+#pragma unroll
+    for (int j = 0; j < EPL; ++j) {
+      const F f = ra[j] * rb[j];
+      s += y ? f * bt[j] : f * ((F)1 - bt[j]);
+      sum += f;
+    }
+    s = group_sum<LPR>(s);
+    sum = group_sum<LPR>(sum);
+    if (!y) s += ((F)1 - sum) * ((F)1 - eps);
+    if (s < (F)1e-30) s = (F)1e-30;
+    // every lane of the row group holds s and rold, hence r
+    const double r = (rold * (cnt - 1.0) + (double)s) / cnt;
+    if (lg == 0) {
+      const double lr = log(r);
+      if (y) {
+        ll1 += lr;
+        n1 += 1.0;
+      } else {
+        ll0 += lr;
+        n0 += 1.0;
+      }
+    }
+    return r;
+  };
+  F rs[kSeq ? UE + 1 : 1][EPL];  // row slots of the sequence walk
+  for (int64_t i0 = 0; w_lo + i0 * RPW < w_hi; i0 += IPL) {
     int32_t pa, pb, py, prl, prh;
     {
-      const int64_t e = base + lane % RPW + (i0 + lane / RPW) * kPer;
-      const int64_t ee = e < e_hi ? e : e_lo;  // (a valid index; not used past e_hi)
-      pa = a.edges[2 * ee];
-      pb = a.edges[2 * ee + 1];
-      py = a.labels[ee] != 0;
+      const int64_t e = w_lo + i0 * RPW + lane;
+      const int64_t ee = e < w_hi ? e : w_lo;  // (a valid index; not used past w_hi)
+      pa = a.seq[ee];
+      pb = a.seq[ee + 1];
+      py = a.labels[ee];
       const double r = a.R[ee];
       prl = (int32_t)__double2loint(r);
       prh = (int32_t)__double2hiint(r);
     }
-    double rnew = 0.0;  // lane j: the new average of its preloaded edge
-    for (int i = 0; i < IPL && base + (i0 + i) * kPer < e_hi; i += UE) {
+    rnew = 0.0;
+    for (int i = 0; i < IPL && w_lo + (i0 + i) * RPW < w_hi; i += UE) {
+      // (every lane takes part in the exchanges: a shuffle reading an
+      // inactive lane returns nothing useful)
       int32_t ea[UE], eb[UE];
       int yv[UE];
       bool val[UE];
       double rold[UE];
-      int64_t ev[UE];
 #pragma unroll
       for (int u = 0; u < UE; ++u) {
         const int src = (i + u) * RPW + g;
-        ev[u] = base + g + (i0 + i + u) * kPer;
-        val[u] = ev[u] < e_hi;
+        val[u] = w_lo + g + (int64_t)(i0 + i + u) * RPW < w_hi;
         ea[u] = take_i(pa, src);
         eb[u] = take_i(pb, src);
         yv[u] = take_i(py, src);
         rold[u] = __hiloint2double(take_i(prh, src), take_i(prl, src));
         if (!val[u]) ea[u] = eb[u] = 0;  // (row 0: in bounds, unused)
       }
-      F ra[UE][EPL], rb[UE][EPL];
+      if constexpr (kSeq) {
+        const bool first = i0 == 0 && i == 0;
+        if (first) {
+          const int32_t s0 = __builtin_amdgcn_readfirstlane(ea[0]);
+          issue_row<F, LPR, CH>(pi + (int64_t)s0 * a.ld, K, lg, rs[0]);
+        }
 #pragma unroll
-      for (int u = 0; u < UE; ++u) {
-        if (LPR == 64) {
-          ea[u] = __builtin_amdgcn_readfirstlane(ea[u]);
+        for (int u = 0; u < UE; ++u) {
           eb[u] = __builtin_amdgcn_readfirstlane(eb[u]);
           yv[u] = __builtin_amdgcn_readfirstlane(yv[u]);
+          issue_row<F, LPR, CH>(pi + (int64_t)eb[u] * a.ld, K, lg, rs[u + 1]);
         }
-        issue_row<F, LPR, CH>(pi + (int64_t)ea[u] * a.ld, K, lg, ra[u]);
-        // row a is shared by neighbouring held-out pairs (sorted by a); row b is streamed
-        issue_row<F, LPR, CH, true>(pi + (int64_t)eb[u] * a.ld, K, lg, rb[u]);
-      }
+        if (first) mask_row<F, LPR, CH>(K, lg, rs[0]);
 #pragma unroll
-      for (int u = 0; u < UE; ++u) {
-        if (!val[u]) continue;  // uniform within the row group
-        mask_row<F, LPR, CH>(K, lg, ra[u]);
-        mask_row<F, LPR, CH>(K, lg, rb[u]);
-        const bool y = yv[u] != 0;
-        F s = (F)0, sum = (F)0;
+        for (int u = 0; u < UE; ++u) {
+          if (!val[u]) continue;  // uniform
+          mask_row<F, LPR, CH>(K, lg, rs[u + 1]);
+          if (yv[u] == kPpxBridge) continue;
+          const double r = pair_ll(rs[u], rs[u + 1], yv[u] != 0, rold[u]);
+          if (lane == i + u) rnew = r;  // (one row group: uniform control flow)
+        }
+        // (a batch with val[UE - 1] false is the wavefront's last)
 #pragma unroll
-        for (int j = 0; j < EPL; ++j) {
-          const F f = ra[u][j] * rb[u][j];
-          s += y ? f * bt[j] : f * ((F)1 - bt[j]);
-          sum += f;
+        for (int j = 0; j < EPL; ++j) rs[0][j] = rs[UE][j];
+      } else {
+        F ra[UE][EPL], rb[UE][EPL];
+#pragma unroll
+        for (int u = 0; u < UE; ++u) {
+          // (a bridge position reads row 0 twice, from L2, and is not counted)
+          if (yv[u] == kPpxBridge) ea[u] = eb[u] = 0;
+          if (LPR == 64) {
+            ea[u] = __builtin_amdgcn_readfirstlane(ea[u]);
+            eb[u] = __builtin_amdgcn_readfirstlane(eb[u]);
+            yv[u] = __builtin_amdgcn_readfirstlane(yv[u]);
+          }
+          issue_row<F, LPR, CH>(pi + (int64_t)ea[u] * a.ld, K, lg, ra[u]);
+          issue_row<F, LPR, CH>(pi + (int64_t)eb[u] * a.ld, K, lg, rb[u]);
         }
-        s = group_sum<LPR>(s);
-        sum = group_sum<LPR>(sum);
-        if (!y) s += ((F)1 - sum) * ((F)1 - eps);
-        if (s < (F)1e-30) s = (F)1e-30;
-        // every lane of the row group holds s and rold, hence r
-        const double r = (rold[u] * (cnt - 1.0) + (double)s) / cnt;
-        if constexpr (RPW == 1) {
-          if (lane == i + u) rnew = r;
-        } else {
-          const double rv = __shfl(r, (lane % RPW) * LPR, 64);
-          if (lane / RPW == i + u) rnew = rv;
-        }
-        if (lg == 0) {
-          const double lr = log(r);
-          if (y) {
-            ll1 += lr;
-            n1 += 1.0;
+#pragma unroll
+        for (int u = 0; u < UE; ++u) {
+          double r = 0.0;
+          if (val[u] && yv[u] != kPpxBridge) {  // uniform within the row group
+            mask_row<F, LPR, CH>(K, lg, ra[u]);
+            mask_row<F, LPR, CH>(K, lg, rb[u]);
+            r = pair_ll(ra[u], rb[u], yv[u] != 0, rold[u]);
+          }
+          // lane j keeps the average of position i0 + j / RPW, row group
+          // j % RPW; every lane takes part (the row groups diverge above)
+          if constexpr (RPW == 1) {
+            if (lane == i + u) rnew = r;
           } else {
-            ll0 += lr;
-            n0 += 1.0;
+            const double rv = __shfl(r, (lane % RPW) * LPR, 64);
+            if (lane / RPW == i + u) rnew = rv;
           }
         }
       }
     }
     {
-      const int64_t e = base + lane % RPW + (i0 + lane / RPW) * kPer;
-      if (e < e_hi) a.R[e] = rnew;
+      const int64_t e = w_lo + i0 * RPW + lane;
+      if (e < w_hi) a.R[e] = rnew;
     }
   }
   // lanes with lg != 0 hold zeros
