@@ -570,6 +570,13 @@ def test_error_paths():
         smp.update_phi([N + 5], np.zeros((1, 4), np.int32), None, 0.01)
     with pytest.raises(M.AmmsbError):
         M.MCMCSamplerStochastic(M.Options(K=K, num_node_sample=N), g, s)
+    # a caller's split with a node id outside [0, N)
+    for bad in (N, -1):
+        hoe = np.array(s.held_out_edges, dtype=np.int32, copy=True)
+        hoe[len(hoe) // 2, 1] = bad
+        bad_split = M.Split(hoe, s.held_out_labels, s.test_edges, s.test_labels)
+        with pytest.raises(M.AmmsbError):
+            M.MCMCSamplerStochastic(M.Options(K=K, num_node_sample=8), g, bad_split)
     # the largest rows the device kernels take: K 2048 (FP64), 4096 (FP32)
     for k_max, prec in ((2048, "fp64"), (4096, "fp32")):
         M.MCMCSamplerStochastic(M.Options(K=k_max, num_node_sample=8, precision=prec), g, s).close()
