@@ -1003,7 +1003,7 @@ __device__ __forceinline__ void phi_epilogue(const PhiArgs& a, int slot, int32_t
   const double inv_s = 1.0 / s;
   // one non-temporal 16-byte store per vector (the row is 16-byte aligned;
   // streaming stores cost the concurrent row gathers of other workgroups
-  // less: C3 FP32 update_phi 383 -> 373 us); a vector that reaches past the
+  // less: N 4M K 1024 FP32 update_phi 383 -> 373 us); a vector that reaches past the
   // row is stored element by element, so that phi_sum at K is written once
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
@@ -1324,7 +1324,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   const T* src = reinterpret_cast<const T*>((const F*)a.staging + (int64_t)slot * a.ld);
   T* dst = reinterpret_cast<T*>((F*)a.pi + (int64_t)nodes[slot] * a.ld);
   // non-temporal stores: the rows are read again only by the next iteration
-  // (C3 FP32: the next update_phi 373 -> 368 us against plain stores)
+  // (N 4M, K 1024, FP32: the next update_phi 373 -> 368 us against plain stores)
   for (int i = lane; i < nv; i += 64) __builtin_nontemporal_store(src[i], dst + i);
 }
 
@@ -1615,7 +1615,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_ppx(PpxArgs a) {
   // requesting the rows seq[t + 1 .. t + UE] into slots 1..UE, while slot 0
   // holds seq[t] (the last row of the previous batch): every row of the
   // sequence is requested once, and pair u is slots (u, u + 1).  UE = 8 up
-  // to 4 vectors per lane, else 2 (C3 FP64, 8 vectors: 12.3 ms per
+  // to 4 vectors per lane, else 2 (N 4M, K 1024, FP64, 8 vectors: 12.3 ms per
   // evaluation, 12.2 with UE = 4 at one wavefront per SIMD, 13.9 requesting
   // both rows of each pair).  Otherwise (several row groups per wavefront
   // for short rows; for longer rows the slots would cost a wavefront per
