@@ -1610,20 +1610,20 @@ __global__ __launch_bounds__(WAVES * 64) void k_ppx(PpxArgs a) {
   // layout and written together after those positions (a store in the loop
   // would be waited for with the next load).
   //
-  // One row group per wavefront and up to 8 vectors per lane: the
-  // wavefront walks its part of the sequence UE positions at a time,
-  // requesting the rows seq[t + 1 .. t + UE] into slots 1..UE, while slot 0
-  // holds seq[t] (the last row of the previous batch): every row of the
-  // sequence is requested once, and pair u is slots (u, u + 1).  UE = 8 up
-  // to 4 vectors per lane, else 2 (N 4M, K 1024, FP64, 8 vectors: 12.3 ms per
-  // evaluation, 12.2 with UE = 4 at one wavefront per SIMD, 13.9 requesting
-  // both rows of each pair).  Otherwise (several row groups per wavefront
-  // for short rows; for longer rows the slots would cost a wavefront per
-  // SIMD) row group g takes positions w_lo + g + i RPW, both rows of each
-  // requested (the row shared with the previous position is found in L2),
-  // UE pairs in flight.
-  constexpr bool kSeq = LPR == 64 && CH <= 8;
-  constexpr int UE = kSeq ? (CH <= 4 ? 8 : 2) : (CH <= 8 ? 2 : 1);
+  // One row group per wavefront: the wavefront walks its part of the
+  // sequence UE positions at a time, requesting the rows seq[t + 1 .. t +
+  // UE] into slots 1..UE, while slot 0 holds seq[t] (the last row of the
+  // previous batch): every row of the sequence is requested once, and pair
+  // u is slots (u, u + 1).  UE = 8 up to 4 vectors per lane, else 2 (N 4M,
+  // K 1024, FP64, 8 vectors: 12.3 ms per evaluation, 12.2 with UE = 4 at one
+  // wavefront per SIMD, 13.9 requesting both rows of each pair; above 8
+  // vectors the slots cost a wavefront per SIMD and still gain: N 400k,
+  // FP32 K 2560 1.55 against 1.86 ms, FP64 K 1200 1.49 against 1.68 ms).
+  // Several row groups per wavefront (short rows): row group g takes
+  // positions w_lo + g + i RPW, both rows of each requested (the row shared
+  // with the previous position is found in L2), two pairs in flight.
+  constexpr bool kSeq = LPR == 64;
+  constexpr int UE = kSeq ? (CH <= 4 ? 8 : 2) : 2;
   constexpr int64_t kAlign = (int64_t)WAVES * RPW;
   constexpr int IPL = 64 / RPW;  // positions per row group and preload
   const int64_t chunk = ((a.H + a.blocks - 1) / a.blocks + kAlign - 1) / kAlign * kAlign;
